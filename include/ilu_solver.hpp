@@ -1,0 +1,99 @@
+// ILU preconditioner on the GPU.
+//
+// Drop-in for include/ilu_solver.hpp of lljbash/par_ilu0_cpp (lines 8-30).
+// The user fills the factor matrix F through GetMatrix() (CopyCsrMatrix for
+// ILU(0), IlukSymbolic for ILU(k)), then calls SetupMatrix() once per
+// sparsity pattern and Factorize() once per value set.  Factorize() leaves
+// the factors in F.value in place, as the reference does
+// (src/ilu_solver.cpp:553-589): L strictly lower with an implied unit
+// diagonal, U upper including the diagonal.
+//
+// The work happens on the current HIP device.  Host-pointer methods copy in
+// and out; the *Device methods work on device-resident data and are the
+// extension surface used by the GMRES solver and by batched callers.
+#pragma once
+
+#include <string>
+#include "csr_matrix.h"
+
+namespace lljbash {
+
+class IluSolver {
+public:
+    IluSolver() = default;
+    ~IluSolver();
+    IluSolver(const IluSolver&) = delete;
+    IluSolver& operator=(const IluSolver&) = delete;
+
+    const CsrMatrix& GetILUMatrix() const { return aMatrix_; }
+    CsrMatrix* GetMatrix() { return &aMatrix_; }
+
+    // Kept for interface compatibility: the reference sets its OpenMP team
+    // size here (src/ilu_solver.cpp:228-237).  The GPU path ignores it.
+    void SetThreads(int threads) { threads_ = threads; }
+
+    // Uploads the pattern, locates diagonals and builds the level schedules
+    // of the L and U dependency graphs.  Throws std::runtime_error("missing
+    // diag") if a row lacks its diagonal entry, std::runtime_error for other
+    // failures.  reference: src/ilu_solver.cpp:225-373
+    void SetupMatrix();
+
+    // Numeric ILU on F's pattern; uploads F.value, factorizes on the GPU and
+    // copies the factors back into F.value.  Returns false if a zero pivot
+    // was met (the factors then hold inf/nan, as the reference's would).
+    // reference: src/ilu_solver.cpp:616-671
+    bool Factorize();
+
+    // x = U^-1 L^-1 b with host vectors; b may alias x.
+    // reference: src/ilu_solver.cpp:736-831
+    void Substitute(const double* b, double* x);
+
+    // ---- extension surface (no counterpart in the reference) ----------
+    // Factorize from values already on the device (length nnz(F)); factors
+    // stay on the device, F.value on the host is not touched.
+    bool FactorizeDevice(const double* d_values);
+    // Apply the preconditioner to device vectors; b may alias x.
+    void SubstituteDevice(const double* d_b, double* d_x);
+    // 64 (default) or 32: precision in which the factors are stored for the
+    // triangular sweeps.  Vectors and accumulation stay FP64.
+    void SetFactorPrecision(int bits);
+    int FactorPrecision() const;
+    // Lanes cooperating on one row in the L and U sweeps (1..64, power of
+    // two); 0 picks a value from the average row length.
+    void SetSweepLanes(int lanes_l, int lanes_u);
+    // Device pointers of the current pattern and factors (valid after
+    // SetupMatrix / Factorize): row_ptr, col_idx, diag_ptr, factor values.
+    const int* DeviceRowPtr() const;
+    const int* DeviceColIdx() const;
+    const int* DeviceDiagPtr() const;
+    const double* DeviceFactors() const;
+    int NumLevelsL() const;
+    int NumLevelsU() const;
+    // Make the solver use the pattern of a device-resident F; the pattern
+    // arrays are copied.  Same checks and schedules as SetupMatrix().
+    void SetupMatrixDevice(int n, int nnz, const int* d_row_ptr, const int* d_col_idx);
+    // Symbolic ILU(lof) of a device-resident A straight into this solver
+    // (pattern, schedules and the A-to-F value map stay on the device);
+    // returns nnz(F), or -1 for a row of A with a repeated column.
+    int SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx, int lof);
+    // Factorize from A's device values through the map kept by
+    // SymbolicDevice (fill entries start at zero).
+    bool FactorizeDeviceFromA(const double* d_a_values);
+    // Copy the level of every entry of F (from the last SymbolicDevice) to
+    // the host; returns nnz(F).
+    int CopyFillLevels(signed char* out) const;
+    int Size() const;
+    int Nonzeros() const;
+
+    struct Ext;
+    // Library-internal device state (opaque to users).
+    Ext* DeviceState() const { return ext_; }
+private:
+    int              threads_ = 0;
+    CsrMatrix        aMatrix_;
+    Ext* ext_ = nullptr;
+    struct ThreadLocalExt;
+    ThreadLocalExt* extt_ = nullptr;  // unused; keeps the reference's layout
+};
+
+} // namespace lljbash

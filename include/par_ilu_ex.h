@@ -1,0 +1,85 @@
+/* Extension C ABI of the MI355X build (no counterpart in the reference).
+ *
+ * These LLJBASH_*Ex entry points add what the reference interface cannot
+ * express: device-resident data, FP32 factor storage, sweep launch shapes
+ * and diagnostics.  The reference's own structs and functions are left
+ * unchanged.  "Device" pointers are HIP device allocations on the current
+ * device; work is ordered on the solver's stream (the null stream), so
+ * data produced by earlier null-stream work is ready.  Return values: 0 (or
+ * a count) on success, -1 on failure, with a message on stderr.
+ */
+#ifndef LLJBASH_PAR_ILU_EX_H_
+#define LLJBASH_PAR_ILU_EX_H_
+
+#include "gmres_c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 64 (default) or 32: precision of the stored L/U factors used by the
+ * sweeps; vectors, Arnoldi and the residual stay FP64. */
+#define IluSolverSetFactorPrecisionEx LLJBASH_DECL(IluSolverSetFactorPrecisionEx)
+int IluSolverSetFactorPrecisionEx(IluSolverHdl hdl, int bits);
+
+/* Lanes per row of the forward / backward sweep (0 = heuristic, else a
+ * power of two <= 64). */
+#define IluSolverSetSweepLanesEx LLJBASH_DECL(IluSolverSetSweepLanesEx)
+int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
+
+/* Number of levels of the L and U dependency graphs (after setup). */
+#define IluSolverGetLevelsEx LLJBASH_DECL(IluSolverGetLevelsEx)
+int IluSolverGetLevelsEx(IluSolverHdl hdl, int* nlev_l, int* nlev_u);
+
+/* Symbolic ILU(lof) of a device CSR `d_a` (row_ptr/col_idx device arrays,
+ * size = n) into the solver, followed by the same analysis as Setup.
+ * Returns nnz(F), -1 on a repeated column, -2 on a missing diagonal. */
+#define IluSolverSymbolicDeviceEx LLJBASH_DECL(IluSolverSymbolicDeviceEx)
+int IluSolverSymbolicDeviceEx(IluSolverHdl hdl, const CsrMatrix* d_a, int lof);
+
+/* Factorize on the device.  values_are_a != 0: d_values are A's values
+ * (nnz(A), mapped through the SymbolicDeviceEx map, fill = 0); otherwise
+ * d_values are F's values (nnz(F)).  Returns 0, or -1 (also on a zero
+ * pivot). */
+#define IluSolverFactorizeDeviceEx LLJBASH_DECL(IluSolverFactorizeDeviceEx)
+int IluSolverFactorizeDeviceEx(IluSolverHdl hdl, const double* d_values, int values_are_a);
+
+/* x = U^-1 L^-1 b on device vectors (b may alias x). */
+#define IluSolverSubstituteDeviceEx LLJBASH_DECL(IluSolverSubstituteDeviceEx)
+int IluSolverSubstituteDeviceEx(IluSolverHdl hdl, const double* d_b, double* d_x);
+
+/* Copies of the device factors (nnz(F) doubles) and fill levels
+ * (nnz(F) bytes, after SymbolicDeviceEx) to host memory. */
+#define IluSolverCopyFactorsEx LLJBASH_DECL(IluSolverCopyFactorsEx)
+int IluSolverCopyFactorsEx(IluSolverHdl hdl, double* out);
+#define IluSolverCopyFillLevelsEx LLJBASH_DECL(IluSolverCopyFillLevelsEx)
+int IluSolverCopyFillLevelsEx(IluSolverHdl hdl, signed char* out);
+#define IluSolverNonzerosEx LLJBASH_DECL(IluSolverNonzerosEx)
+int IluSolverNonzerosEx(IluSolverHdl hdl);
+
+/* Symbolic ILU(lof) on host CSR like IlukSymbolic, also returning the level
+ * of every entry of F in `levels` (caller-provided, capacity `cap`). */
+#define IlukSymbolicLevelsEx LLJBASH_DECL(IlukSymbolicLevelsEx)
+int IlukSymbolicLevelsEx(const CsrMatrix* a, int lof, CsrMatrix* f, int* nzmap, signed char* levels, long cap);
+
+/* GMRES on device data (d_mat arrays on the device). */
+#define GmresSolveDeviceEx LLJBASH_DECL(GmresSolveDeviceEx)
+bool GmresSolveDeviceEx(GmresHdl hdl, const CsrMatrix* d_mat, const double* d_rhs, double* d_sol, int* iter);
+
+/* ||b - A x||_2 of the last solve's result (FP64, on the device). */
+#define GmresLastTrueResidualEx LLJBASH_DECL(GmresLastTrueResidualEx)
+double GmresLastTrueResidualEx(GmresHdl hdl);
+
+/* Residual estimates |rs[i+1]| after each Arnoldi step of the last solve. */
+#define GmresResidualHistoryEx LLJBASH_DECL(GmresResidualHistoryEx)
+int GmresResidualHistoryEx(GmresHdl hdl, double* out, int cap);
+
+/* 1 if a HIP device is usable from this process, else 0. */
+#define DeviceAvailableEx LLJBASH_DECL(DeviceAvailableEx)
+int DeviceAvailableEx(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LLJBASH_PAR_ILU_EX_H_ */

@@ -1,0 +1,142 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py's cpu_baseline leg.  The product package never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIBS: dict[str, C.CDLL] = {}
+
+_I = C.POINTER(C.c_int)
+_D = C.POINTER(C.c_double)
+
+
+def _load(fast: bool = False) -> C.CDLL:
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = HERE / name
+    if not path.exists():
+        raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+    lib = C.CDLL(str(path))
+    lib.orc_iluk_symbolic.restype = C.c_int
+    lib.orc_iluk_symbolic.argtypes = [C.c_int, _I, _I, _D, C.c_int, C.POINTER(_I), C.POINTER(_I),
+                                      C.POINTER(_I), C.POINTER(_D), _I]
+    lib.orc_free.argtypes = [C.c_void_p]
+    lib.orc_diag_ptr.argtypes = [C.c_int, _I, _I, _I]
+    lib.orc_ilu_factorize.restype = C.c_int
+    lib.orc_ilu_factorize.argtypes = [C.c_int, _I, _I, _I, _D]
+    lib.orc_substitute.argtypes = [C.c_int, _I, _I, _I, _D, _D, _D]
+    lib.orc_spmv.argtypes = [C.c_int, _I, _I, _D, C.c_double, _D, C.c_double, _D]
+    lib.orc_gmres.restype = C.c_int
+    lib.orc_gmres.argtypes = [C.c_int, _I, _I, _D, _I, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int,
+                              _I, _D, C.c_int, _I]
+    _LIBS[name] = lib
+    return lib
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_I)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_D)
+
+
+def iluk_symbolic(row_ptr, col_idx, value, lof: int, fast: bool = False):
+    """-> (frp, fci, flev, fval, nzmap) or None when nnz(F) would not be
+    defined (repeated column)."""
+    lib = _load(fast)
+    rp, ci = _i(row_ptr), _i(col_idx)
+    va = _d(value) if value is not None else None
+    n = rp.size - 1
+    frp, fci, flev = _I(), _I(), _I()
+    fval = _D()
+    nzmap = np.zeros(max(int(rp[-1]), 1), dtype=np.int32)
+    nnz = lib.orc_iluk_symbolic(n, _pi(rp), _pi(ci), _pd(va) if va is not None else None, int(lof),
+                                C.byref(frp), C.byref(fci), C.byref(flev), C.byref(fval), _pi(nzmap))
+    if nnz < 0:
+        return None
+    out = (
+        np.ctypeslib.as_array(frp, shape=(n + 1,)).copy(),
+        np.ctypeslib.as_array(fci, shape=(max(nnz, 1),))[:nnz].copy(),
+        np.ctypeslib.as_array(flev, shape=(max(nnz, 1),))[:nnz].copy(),
+        np.ctypeslib.as_array(fval, shape=(max(nnz, 1),))[:nnz].copy(),
+        nzmap[: int(rp[-1])].copy(),
+    )
+    for p in (frp, fci, flev, fval):
+        lib.orc_free(C.cast(p, C.c_void_p))
+    return out
+
+
+def diag_ptr(row_ptr, col_idx):
+    lib = _load()
+    rp, ci = _i(row_ptr), _i(col_idx)
+    n = rp.size - 1
+    dg = np.empty(n, dtype=np.int32)
+    lib.orc_diag_ptr(n, _pi(rp), _pi(ci), _pi(dg))
+    return dg
+
+
+def ilu_factorize(row_ptr, col_idx, value, fast: bool = False):
+    """-> (factors, diag, zero_pivots)."""
+    lib = _load(fast)
+    rp, ci = _i(row_ptr), _i(col_idx)
+    val = _d(value).copy()
+    dg = diag_ptr(rp, ci)
+    z = lib.orc_ilu_factorize(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(val))
+    return val, dg, z
+
+
+def substitute(row_ptr, col_idx, diag, factors, b, fast: bool = False):
+    lib = _load(fast)
+    rp, ci, dg, fv = _i(row_ptr), _i(col_idx), _i(diag), _d(factors)
+    bb = _d(b)
+    x = np.empty_like(bb)
+    lib.orc_substitute(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(fv), _pd(bb), _pd(x))
+    return x
+
+
+def spmv(row_ptr, col_idx, value, x, alpha=1.0, beta=0.0, y=None):
+    lib = _load()
+    rp, ci, va, xx = _i(row_ptr), _i(col_idx), _d(value), _d(x)
+    yy = np.zeros(rp.size - 1) if y is None else _d(y).copy()
+    lib.orc_spmv(rp.size - 1, _pi(rp), _pi(ci), _pd(va), float(alpha), _pd(xx), float(beta), _pd(yy))
+    return yy
+
+
+def gmres(a, rhs, tol, m, maxits, factors=None, x0=None, fast: bool = False):
+    """Restarted GMRES(m) of the reference.  a = (row_ptr, col_idx, value);
+    factors = (row_ptr, col_idx, diag, value) of the ILU or None (M = I).
+    -> (ok, its, x, history)."""
+    lib = _load(fast)
+    arp, aci, ava = _i(a[0]), _i(a[1]), _d(a[2])
+    n = arp.size - 1
+    b = _d(rhs)
+    x = np.zeros(n) if x0 is None else _d(x0).copy()
+    its = C.c_int(0)
+    nh = C.c_int(0)
+    hist = np.zeros(maxits + 2)
+    if factors is not None:
+        frp, fci, fdg, fva = _i(factors[0]), _i(factors[1]), _i(factors[2]), _d(factors[3])
+        fp = (_pi(frp), _pi(fci), _pi(fdg), _pd(fva))
+    else:
+        fp = (None, None, None, None)
+    ok = lib.orc_gmres(n, _pi(arp), _pi(aci), _pd(ava), *fp, _pd(b), _pd(x), float(tol), int(m), int(maxits),
+                       C.byref(its), _pd(hist), hist.size, C.byref(nh))
+    return bool(ok == 1), its.value, x, hist[: nh.value].copy()

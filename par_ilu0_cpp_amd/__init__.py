@@ -1,0 +1,270 @@
+"""MI355X (gfx950) ILU(k)-preconditioned restarted GMRES.
+
+Python host mirror of the reference interface of lljbash/par_ilu0_cpp over
+this project's C ABI (include/*.h).  Names follow the reference's C++ API
+(include/ilu_solver.hpp:8-30, include/gmres.hpp:9-22, include/iluk.h:15) so
+that tests read like the reference's own drivers (test/test.c,
+test/benchmark.cpp).  All numerical work runs in the HIP libraries built from
+``csrc/``; nothing here computes a result.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import CsrMatrix, GmresParameters, GmresStat, fn, lib
+
+__all__ = [
+    "CsrMatrix",
+    "GmresParameters",
+    "GmresStat",
+    "HostCsr",
+    "IluSolver",
+    "PreconditionedGmres",
+    "IlukSymbolic",
+    "iluk_symbolic_levels",
+    "device_available",
+    "lib",
+]
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _view(ptr, count: int, dtype) -> np.ndarray:
+    if count <= 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(count,)).view(dtype)
+
+
+def csr_to_numpy(m: CsrMatrix):
+    """Copies (row_ptr, col_idx, value) out of a library CsrMatrix."""
+    n = m.size
+    rp = _view(m.row_ptr, n + 1, np.int32).copy()
+    nnz = int(rp[-1])
+    return rp, _view(m.col_idx, nnz, np.int32).copy(), _view(m.value, nnz, np.float64).copy()
+
+
+def fill_csr(m: CsrMatrix, row_ptr, col_idx, value) -> None:
+    """SetupCsrMatrix + copy of numpy arrays into library-owned storage."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+    va = np.ascontiguousarray(value, dtype=np.float64)
+    n = rp.size - 1
+    nnz = int(rp[-1])
+    fn("SetupCsrMatrix")(C.byref(m), n, nnz)
+    C.memmove(m.row_ptr, rp.ctypes.data, rp.nbytes)
+    if nnz:
+        C.memmove(m.col_idx, ci.ctypes.data, nnz * 4)
+        C.memmove(m.value, va.ctypes.data, nnz * 8)
+
+
+class HostCsr:
+    """A library-owned host CsrMatrix (released by DestroyCsrMatrix)."""
+
+    def __init__(self, row_ptr=None, col_idx=None, value=None):
+        self.mat = CsrMatrix(-1, 0, None, None, None)
+        if row_ptr is not None:
+            fill_csr(self.mat, row_ptr, col_idx, value)
+
+    @classmethod
+    def from_csr(cls, csr) -> "HostCsr":
+        return cls(csr.row_ptr, csr.col_idx, csr.value)
+
+    @classmethod
+    def read_mm1(cls, path: str) -> "HostCsr":
+        h = cls()
+        fn("ReadCsrMatrixMM1")(C.byref(h.mat), path.encode())
+        return h
+
+    @property
+    def size(self) -> int:
+        return self.mat.size
+
+    @property
+    def nnz(self) -> int:
+        return int(self.mat.row_ptr[self.mat.size]) if self.mat.size >= 0 and self.mat.row_ptr else 0
+
+    def arrays(self):
+        return csr_to_numpy(self.mat)
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.size, dtype=np.float64)
+        fn("CsrMatVec")(C.byref(self.mat), _ptr(x, C.c_double), _ptr(y, C.c_double))
+        return y
+
+    def destroy(self) -> None:
+        if self.mat.size >= 0 or self.mat.row_ptr:
+            fn("DestroyCsrMatrix")(C.byref(self.mat))
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def IlukSymbolic(a: HostCsr, lof: int, f: CsrMatrix, nzmap: np.ndarray | None = None) -> int:
+    """include/iluk.h (reference include/iluk.h:15): returns nnz(F)."""
+    p = _ptr(nzmap, C.c_int) if nzmap is not None else None
+    return int(fn("IlukSymbolic")(C.byref(a.mat), int(lof), C.byref(f), p))
+
+
+def iluk_symbolic_levels(a: HostCsr, lof: int):
+    """Symbolic ILU(lof) on the GPU returning (row_ptr, col_idx, value,
+    levels, nzmap) of F as numpy arrays."""
+    f = HostCsr()
+    nzmap = np.zeros(max(a.nnz, 1), dtype=np.int32)
+    nnzf = IlukSymbolic(a, lof, f.mat, nzmap)
+    if nnzf < 0:
+        raise RuntimeError("IlukSymbolic failed")
+    lev = np.zeros(max(nnzf, 1), dtype=np.int8)
+    rc = fn("IlukSymbolicLevelsEx")(
+        C.byref(a.mat), int(lof), C.byref(f.mat), _ptr(nzmap, C.c_int), _ptr(lev, C.c_byte), lev.size
+    )
+    if rc != nnzf:
+        raise RuntimeError(f"IlukSymbolicLevelsEx returned {rc}, expected {nnzf}")
+    rp, ci, va = f.arrays()
+    return rp, ci, va, lev[:nnzf].copy(), nzmap[: a.nnz].copy()
+
+
+class IluSolver:
+    """lljbash::IluSolver through the C ABI (include/ilu_solver_c.h)."""
+
+    def __init__(self, nthreads: int = 0):
+        self.hdl = fn("IluSolverCreate")(int(nthreads))
+
+    def GetMatrix(self) -> CsrMatrix:
+        return fn("IluSolverGetMatrix")(self.hdl).contents
+
+    def SetMatrix(self, row_ptr, col_idx, value) -> None:
+        """Fill F (the reference's CopyCsrMatrix(GetMatrix(), &a))."""
+        fill_csr(self.GetMatrix(), row_ptr, col_idx, value)
+
+    def SetupMatrix(self) -> int:
+        return int(fn("IluSolverSetup")(self.hdl))
+
+    def Factorize(self) -> int:
+        return int(fn("IluSolverFactorize")(self.hdl))
+
+    def Substitute(self, b: np.ndarray) -> np.ndarray:
+        """x = U^-1 L^-1 b through the device sweeps (host vectors)."""
+        import torch
+
+        bt = torch.as_tensor(np.ascontiguousarray(b, dtype=np.float64)).cuda()
+        xt = torch.empty_like(bt)
+        self.SubstituteDevice(bt.data_ptr(), xt.data_ptr())
+        torch.cuda.synchronize()
+        return xt.cpu().numpy()
+
+    def SubstituteDevice(self, d_b: int, d_x: int) -> None:
+        if fn("IluSolverSubstituteDeviceEx")(self.hdl, C.c_void_p(d_b), C.c_void_p(d_x)) != 0:
+            raise RuntimeError("IluSolverSubstituteDeviceEx failed")
+
+    def SetFactorPrecision(self, bits: int) -> None:
+        if fn("IluSolverSetFactorPrecisionEx")(self.hdl, int(bits)) != 0:
+            raise RuntimeError("bad precision")
+
+    def SetSweepLanes(self, lanes_l: int, lanes_u: int) -> None:
+        if fn("IluSolverSetSweepLanesEx")(self.hdl, int(lanes_l), int(lanes_u)) != 0:
+            raise RuntimeError("bad lanes")
+
+    def Levels(self) -> tuple[int, int]:
+        a, b = C.c_int(0), C.c_int(0)
+        fn("IluSolverGetLevelsEx")(self.hdl, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def SymbolicDevice(self, n: int, d_row_ptr: int, d_col_idx: int, lof: int) -> int:
+        m = CsrMatrix(n, 0, C.cast(C.c_void_p(d_row_ptr), C.POINTER(C.c_int)),
+                      C.cast(C.c_void_p(d_col_idx), C.POINTER(C.c_int)), None)
+        return int(fn("IluSolverSymbolicDeviceEx")(self.hdl, C.byref(m), int(lof)))
+
+    def FactorizeDevice(self, d_values: int, values_are_a: bool) -> int:
+        return int(fn("IluSolverFactorizeDeviceEx")(self.hdl, C.c_void_p(d_values), int(values_are_a)))
+
+    def Nonzeros(self) -> int:
+        return int(fn("IluSolverNonzerosEx")(self.hdl))
+
+    def DeviceFactors(self) -> np.ndarray:
+        out = np.empty(max(self.Nonzeros(), 1), dtype=np.float64)
+        rc = fn("IluSolverCopyFactorsEx")(self.hdl, _ptr(out, C.c_double))
+        if rc < 0:
+            raise RuntimeError("IluSolverCopyFactorsEx failed")
+        return out[:rc]
+
+    def FillLevels(self) -> np.ndarray:
+        out = np.empty(max(self.Nonzeros(), 1), dtype=np.int8)
+        rc = fn("IluSolverCopyFillLevelsEx")(self.hdl, _ptr(out, C.c_byte))
+        return out[:rc]
+
+    def close(self) -> None:
+        if self.hdl:
+            fn("IluSolverDestroy")(self.hdl)
+            self.hdl = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PreconditionedGmres:
+    """lljbash::PreconditionedGmres through the C ABI (include/gmres_c.h)."""
+
+    def __init__(self):
+        self.hdl = fn("GmresCreate")()
+        self._ilu = None
+
+    def param(self) -> GmresParameters:
+        return fn("GmresGetParameters")(self.hdl).contents
+
+    def stat(self) -> GmresStat:
+        return fn("GmresGetStat")(self.hdl).contents
+
+    def SetIluPreconditioner(self, ilu: IluSolver) -> None:
+        self._ilu = ilu
+        fn("GmresSetPreconditioner")(self.hdl, ilu.hdl)
+
+    def Solve(self, a: HostCsr, rhs: np.ndarray, sol: np.ndarray) -> tuple[bool, int]:
+        """sol (float64, contiguous) is the initial guess and receives x."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        assert sol.dtype == np.float64 and sol.flags.c_contiguous
+        it = C.c_int(0)
+        ok = fn("GmresSolve")(self.hdl, C.byref(a.mat), _ptr(rhs, C.c_double), _ptr(sol, C.c_double), C.byref(it))
+        return bool(ok), it.value
+
+    def SolveDevice(self, n: int, d_row_ptr: int, d_col_idx: int, d_value: int, d_rhs: int, d_sol: int):
+        m = CsrMatrix(n, 0, C.cast(C.c_void_p(d_row_ptr), C.POINTER(C.c_int)),
+                      C.cast(C.c_void_p(d_col_idx), C.POINTER(C.c_int)),
+                      C.cast(C.c_void_p(d_value), C.POINTER(C.c_double)))
+        it = C.c_int(0)
+        ok = fn("GmresSolveDeviceEx")(self.hdl, C.byref(m), C.c_void_p(d_rhs), C.c_void_p(d_sol), C.byref(it))
+        return bool(ok), it.value
+
+    def LastTrueResidual(self) -> float:
+        return float(fn("GmresLastTrueResidualEx")(self.hdl))
+
+    def ResidualHistory(self, cap: int = 100000) -> np.ndarray:
+        out = np.empty(cap, dtype=np.float64)
+        k = fn("GmresResidualHistoryEx")(self.hdl, _ptr(out, C.c_double), cap)
+        return out[: max(k, 0)].copy()
+
+    def close(self) -> None:
+        if self.hdl:
+            fn("GmresDestroy")(self.hdl)
+            self.hdl = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_available() -> bool:
+    return bool(fn("DeviceAvailableEx")())

@@ -1,0 +1,505 @@
+// lljbash::PreconditionedGmres on the GPU.
+// Interface: include/gmres.hpp (reference include/gmres.hpp:9-47); the
+// iteration is src/gmres.cpp:138-288 with all vector work on the device.
+//
+// Everything inside a restart cycle is enqueued without host round trips:
+// the scalar part of each Arnoldi step (step condition, Givens update,
+// breakdown and NaN checks) runs in single-block kernels on a small control
+// block in device memory, and every vector kernel of a step is gated by the
+// control block's step flag.  The host only looks at the control block one
+// step behind (to stop enqueuing) and at the end of each cycle.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "gmres.hpp"
+#include "ilu_internal.hpp"
+
+namespace pilu {
+
+constexpr int kVecBlocks = 1024;  // fixed grid of the reducing vector kernels
+constexpr int kVecThreads = 256;
+
+struct GmresCtl {
+    int its;           // Arnoldi steps so far (the reference's `its`)
+    int maxits;
+    int im;            // restart length m
+    int cycle_active;  // inner loop still running
+    int step_active;   // the current step runs
+    int i;             // last step index of the current cycle
+    int status;        // 0 running, 1 converged at cycle start, 2 NaN estimate
+    int breakdown;     // ||w|| <= 1e-16 in the last step
+    int hist_cap;
+    int pad;
+    double tol;
+    double beta;       // latest residual estimate
+    double inv;        // scale for the vector just normalised
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the block in a fixed order; every thread gets the total.
+__device__ double block_sum(double v) {
+    __shared__ double s_part[kVecThreads / 64];
+    __shared__ double s_total;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) t += s_part[w];
+        s_total = t;
+    }
+    __syncthreads();
+    return s_total;
+}
+
+// Sum of kVecBlocks partials, identical in every block.
+__device__ double sum_partials(const double* part) {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < kVecBlocks; k += blockDim.x) v += part[k];
+    return block_sum(v);
+}
+
+// y = alpha A x + beta y (beta == 0: y not read), G lanes per row; with
+// dotv, part[block] = partial sum of dotv[r] * y[r]; with nrm, partial sum
+// of y[r]^2.
+template <int G>
+__global__ __launch_bounds__(kVecThreads) void k_spmv(int n, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci,
+                                                      const double* __restrict__ val, double alpha,
+                                                      const double* __restrict__ x, double beta,
+                                                      const double* yin, double* y,
+                                                      const double* __restrict__ dotv, bool nrm,
+                                                      double* part, const int* gate) {
+    if (gate && *gate == 0) return;
+    constexpr int R = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int slot = lane / G;
+    const int t = lane & (G - 1);
+    const long step = static_cast<long>(gridDim.x) * (kVecThreads / 64) * R;
+    double dacc = 0.0;
+    for (long row = (static_cast<long>(blockIdx.x) * (kVecThreads / 64) + wave) * R + slot; row < n; row += step) {
+        double s = 0.0;
+        for (int e = rp[row] + t; e < rp[row + 1]; e += G) s = fma(val[e], x[ci[e]], s);
+        s = group_sum<G>(s);
+        if (t == 0) {
+            const double yv = beta == 0.0 ? alpha * s : beta * yin[row] + alpha * s;
+            y[row] = yv;
+            if (dotv) dacc = fma(dotv[row], yv, dacc);
+            else if (nrm) dacc = fma(yv, yv, dacc);
+        }
+    }
+    if (part) {
+        const double tot = block_sum(dacc);
+        if (threadIdx.x == 0) part[blockIdx.x] = tot;
+    }
+}
+
+// One modified Gram-Schmidt step (src/gmres.cpp:219-224):
+//   h = (v_j, w) from the partials of the previous kernel;
+//   w -= h v_j;  then partials of (vnext, w) or of (w, w) if vnext is null.
+__global__ __launch_bounds__(kVecThreads) void k_mgs(int n, const double* __restrict__ vj, double* w,
+                                                     const double* __restrict__ vnext,
+                                                     const double* part_in, double* part_out,
+                                                     double* hout, const int* gate) {
+    if (gate && *gate == 0) return;
+    const double h = sum_partials(part_in);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+    double acc = 0.0;
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long r = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += stride) {
+        const double wr = fma(-h, vj[r], w[r]);
+        w[r] = wr;
+        acc = fma(vnext ? vnext[r] : wr, wr, acc);
+    }
+    const double tot = block_sum(acc);
+    if (threadIdx.x == 0) part_out[blockIdx.x] = tot;
+}
+
+// v *= ctl->inv when the current cycle/step calls for it.
+__global__ __launch_bounds__(kVecThreads) void k_scale(int n, double* v, const GmresCtl* ctl, bool step_gate) {
+    if (step_gate ? (ctl->step_active == 0 || ctl->breakdown) : ctl->cycle_active == 0) return;
+    const double a = ctl->inv;
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long r = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += stride) v[r] *= a;
+}
+
+// Start of a cycle (src/gmres.cpp:185-198).
+__global__ void k_cycle_begin(GmresCtl* ctl, const double* part, double* rs) {
+    const double beta = std::sqrt(sum_partials(part));
+    if (threadIdx.x != 0) return;
+    ctl->beta = beta;
+    ctl->i = -1;
+    ctl->breakdown = 0;
+    ctl->step_active = 0;
+    if (beta <= ctl->tol) {
+        ctl->status = 1;
+        ctl->cycle_active = 0;
+        return;
+    }
+    ctl->inv = 1.0 / beta;
+    rs[0] = beta;
+    ctl->cycle_active = 1;
+}
+
+// Loop condition of the Krylov loop (src/gmres.cpp:199) for step i.
+__global__ void k_step_gate(GmresCtl* ctl, int i) {
+    if (ctl->cycle_active && ctl->beta > ctl->tol) {
+        const int before = ctl->its++;
+        if (before < ctl->maxits) {
+            ctl->step_active = 1;
+            ctl->i = i;
+            return;
+        }
+    }
+    ctl->step_active = 0;
+    ctl->cycle_active = 0;
+}
+
+// End of an Arnoldi step (src/gmres.cpp:226-262): h_{i+1,i} = ||w||,
+// breakdown test, earlier rotations on column i, new rotation, estimate.
+__global__ void k_arnoldi_finish(GmresCtl* ctl, const double* part, double* hh, double* hist) {
+    if (ctl->step_active == 0) return;
+    const double t = std::sqrt(sum_partials(part));
+    if (threadIdx.x != 0) return;
+    const int im = ctl->im, im1 = im + 1;
+    const int i = ctl->i, i1 = i + 1;
+    double* col = hh + static_cast<long>(i) * im1;
+    double* c = hh + static_cast<long>(im1) * im;
+    double* s = c + im1;
+    double* rs = s + im1;
+    col[i1] = t;
+    if (t <= 1e-16) {
+        ctl->breakdown = 1;
+        ctl->cycle_active = 0;
+        return;
+    }
+    ctl->inv = 1.0 / t;
+    for (int k = 1; k <= i; ++k) {
+        const int k1 = k - 1;
+        const double tk = col[k1];
+        col[k1] = c[k1] * tk + s[k1] * col[k];
+        col[k] = -s[k1] * tk + c[k1] * col[k];
+    }
+    const double a = col[i], b = col[i1];
+    double gam = std::sqrt(a * a + b * b);
+    if (gam == 0.0) gam = 1e-16;
+    c[i] = col[i] / gam;
+    s[i] = col[i1] / gam;
+    rs[i1] = -s[i] * rs[i];
+    rs[i] = c[i] * rs[i];
+    col[i] = c[i] * col[i] + s[i] * col[i1];
+    const double beta = std::fabs(rs[i1]);
+    ctl->beta = beta;
+    if (hist && ctl->its - 1 < ctl->hist_cap) hist[ctl->its - 1] = beta;
+    if (std::isnan(beta)) {
+        ctl->status = 2;
+        ctl->cycle_active = 0;
+    }
+}
+
+// Upper-triangular solve for the step coefficients (src/gmres.cpp:265-272).
+__global__ void k_backsolve(GmresCtl* ctl, double* hh) {
+    if (ctl->status == 2 || ctl->i < 0) return;
+    const int im = ctl->im, im1 = im + 1;
+    const int i = ctl->i;
+    double* rs = hh + static_cast<long>(im1) * im + 2 * im1;
+    rs[i] = rs[i] / hh[static_cast<long>(i) * im1 + i];
+    for (int ii = i - 1; ii >= 0; --ii) {
+        double t = rs[ii];
+        for (int j = ii + 1; j <= i; ++j) t -= hh[static_cast<long>(j) * im1 + ii] * rs[j];
+        rs[ii] = t / hh[static_cast<long>(ii) * im1 + ii];
+    }
+}
+
+// sol += sum_j rs_j z_j, j = 0..i, applied in ascending j per entry
+// (src/gmres.cpp:274-276).
+__global__ __launch_bounds__(kVecThreads) void k_update(int n, const double* __restrict__ z, const GmresCtl* ctl,
+                                                        const double* hh, double* sol) {
+    if (ctl->status == 2 || ctl->i < 0) return;
+    const int im = ctl->im, im1 = im + 1;
+    const int i = ctl->i;
+    const double* rs = hh + static_cast<long>(im1) * im + 2 * im1;
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long r = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += stride) {
+        double x = sol[r];
+        for (int j = 0; j <= i; ++j) x = fma(rs[j], z[static_cast<long>(j) * n + r], x);
+        sol[r] = x;
+    }
+}
+
+// A must be a valid CSR before the SpMV kernels read through it:
+// row_ptr[0] == 0, non-decreasing, row_ptr[n] == nnz, columns in [0, n).
+__global__ void k_check_csr(int n, long nnz, const int* __restrict__ rp, const int* __restrict__ ci, int* bad) {
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i <= n; i += stride) {
+        const bool ok = (i == 0) ? rp[0] == 0 : (rp[i] >= rp[i - 1] && (i < n || rp[n] == nnz));
+        if (!ok) atomicOr(bad, 1);
+    }
+}
+
+__global__ void k_check_cols(int n, long nnz, const int* __restrict__ ci, int* bad) {
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long k = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += stride) {
+        const int c = ci[k];
+        if (c < 0 || c >= n) atomicOr(bad, 2);
+    }
+}
+
+__global__ void k_sqrt_sum(const double* part, double* out) {
+    const double v = std::sqrt(sum_partials(part));
+    if (threadIdx.x == 0) *out = v;
+}
+
+}  // namespace pilu
+
+struct lljbash::PreconditionedGmres::Ext {
+    int n = 0, m = 0;
+    pilu::DevBuf<double> vv, z, hh, part, hist, scratch;
+    pilu::DevBuf<int> flags;
+    pilu::DevBuf<pilu::GmresCtl> ctl;
+    pilu::DevBuf<int> arp, aci;
+    pilu::DevBuf<double> aval, rhs, sol;
+    pilu::GmresCtl* snap = nullptr;  // pinned host snapshots (2 slots)
+    hipEvent_t snap_ev[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> tev;     // timing events (pairs)
+    int hist_len = 0;
+    double true_res = -1.0;
+    ~Ext() {
+        if (snap) (void)hipHostFree(snap);
+        for (auto& e : snap_ev) if (e) (void)hipEventDestroy(e);
+        for (auto& e : tev) (void)hipEventDestroy(e);
+    }
+};
+
+namespace lljbash {
+
+namespace {
+
+using GExt = PreconditionedGmres::Ext;
+
+template <int G>
+void spmv_g(hipStream_t s, int n, const int* rp, const int* ci, const double* val, double alpha, const double* x,
+            double beta, const double* yin, double* y, const double* dotv, bool nrm, double* part, const int* gate) {
+    pilu::k_spmv<G><<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(n, rp, ci, val, alpha, x, beta, yin, y, dotv,
+                                                                  nrm, part, gate);
+}
+
+void spmv(int lanes, hipStream_t s, int n, const int* rp, const int* ci, const double* val, double alpha,
+          const double* x, double beta, const double* yin, double* y, const double* dotv, bool nrm, double* part,
+          const int* gate) {
+    switch (lanes) {
+        case 4: spmv_g<4>(s, n, rp, ci, val, alpha, x, beta, yin, y, dotv, nrm, part, gate); break;
+        case 8: spmv_g<8>(s, n, rp, ci, val, alpha, x, beta, yin, y, dotv, nrm, part, gate); break;
+        case 16: spmv_g<16>(s, n, rp, ci, val, alpha, x, beta, yin, y, dotv, nrm, part, gate); break;
+        case 32: spmv_g<32>(s, n, rp, ci, val, alpha, x, beta, yin, y, dotv, nrm, part, gate); break;
+        default: spmv_g<64>(s, n, rp, ci, val, alpha, x, beta, yin, y, dotv, nrm, part, gate); break;
+    }
+}
+
+struct TimedPair {
+    hipEvent_t a, b;
+    int kind;  // 0 = mv, 1 = precon
+};
+
+std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& prm, GmresStat& st, int n,
+                                long nnz, const int* rp, const int* ci, const double* val, const double* rhs,
+                                double* sol) {
+    if (!ilu || !ilu->DeviceState()) return {false, 0};
+    IluSolver::Ext* ie = ilu->DeviceState();
+    if (ie->n != n) throw std::runtime_error("GMRES: matrix size differs from the preconditioner's");
+    hipStream_t s = pilu::ilu_stream(ie);
+    const int m = prm.krylov_subspace_dimension;
+    const int im1 = m + 1;
+    if (m < 1) throw std::runtime_error("GMRES: krylov_subspace_dimension must be >= 1");
+    e->vv.reserve(static_cast<size_t>(im1) * n);
+    e->z.reserve(static_cast<size_t>(m) * n);
+    e->hh.reserve(static_cast<size_t>(im1) * (m + 3));
+    e->part.reserve(2 * pilu::kVecBlocks);
+    e->hist.reserve(static_cast<size_t>(prm.max_iterations) + 2);
+    e->scratch.reserve(n);
+    e->ctl.reserve(1);
+    e->n = n;
+    e->m = m;
+    if (!e->snap) {
+        PILU_HIP(hipHostMalloc(reinterpret_cast<void**>(&e->snap), 2 * sizeof(pilu::GmresCtl)));
+        for (auto& ev : e->snap_ev) PILU_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    const double avg = n > 0 ? static_cast<double>(nnz) / n : 1.0;
+    const int lanes = pilu::lanes_for_avg(avg);
+
+    // Validate A before any kernel indexes through it.
+    e->flags.reserve(4);
+    PILU_HIP(hipMemsetAsync(e->flags.get(), 0, sizeof(int), s));
+    pilu::k_check_csr<<<256, 256, 0, s>>>(n, nnz, rp, ci, e->flags.get());
+    int bad = 0;
+    PILU_HIP(hipMemcpyAsync(&bad, e->flags.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    if (bad) throw std::runtime_error("GMRES: row_ptr of A is not a valid CSR row pointer");
+    pilu::k_check_cols<<<1024, 256, 0, s>>>(n, nnz, ci, e->flags.get());
+    PILU_HIP(hipMemcpyAsync(&bad, e->flags.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    if (bad) throw std::runtime_error("GMRES: column index of A out of range");
+
+    pilu::GmresCtl h{};
+    h.its = 0;
+    h.maxits = prm.max_iterations;
+    h.im = m;
+    h.hist_cap = prm.max_iterations + 2;
+    h.tol = prm.tolerance;
+    PILU_HIP(hipMemcpyAsync(e->ctl.get(), &h, sizeof(h), hipMemcpyHostToDevice, s));
+    PILU_HIP(hipMemsetAsync(e->hh.get(), 0, sizeof(double) * im1 * (m + 3), s));
+
+    pilu::GmresCtl* dctl = e->ctl.get();
+    const int* step_gate = &dctl->step_active;
+    double* vv = e->vv.get();
+    double* z = e->z.get();
+    double* hh = e->hh.get();
+    double* part0 = e->part.get();
+    double* part1 = part0 + pilu::kVecBlocks;
+    std::vector<TimedPair> timed;
+    size_t ev_used = 0;
+    auto next_event = [&]() {
+        if (ev_used == e->tev.size()) {
+            hipEvent_t ev;
+            PILU_HIP(hipEventCreate(&ev));
+            e->tev.push_back(ev);
+        }
+        return e->tev[ev_used++];
+    };
+    auto t_begin = [&](int kind) {
+        TimedPair p{next_event(), next_event(), kind};
+        PILU_HIP(hipEventRecord(p.a, s));
+        timed.push_back(p);
+    };
+    auto t_end = [&]() { PILU_HIP(hipEventRecord(timed.back().b, s)); };
+
+    const auto wall0 = std::chrono::steady_clock::now();
+    bool ok = true;
+    int its = 0;
+    while (its < prm.max_iterations) {
+        // r = rhs - A sol, ||r|| (src/gmres.cpp:181-185)
+        t_begin(0);
+        spmv(lanes, s, n, rp, ci, val, -1.0, sol, 1.0, rhs, vv, nullptr, true, part0, nullptr);
+        t_end();
+        pilu::k_cycle_begin<<<1, pilu::kVecThreads, 0, s>>>(dctl, part0, hh + static_cast<long>(im1) * m + 2 * im1);
+        PILU_HIP(hipMemcpyAsync(&h, dctl, sizeof(h), hipMemcpyDeviceToHost, s));
+        PILU_HIP(hipStreamSynchronize(s));
+        if (h.status == 1) break;  // beta <= tol (src/gmres.cpp:186-188)
+        // A NaN residual would make the reference's outer loop spin forever
+        // (its inner loop never runs); report failure instead.
+        if (std::isnan(h.beta)) { ok = false; break; }
+        pilu::k_scale<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(n, vv, dctl, false);
+        for (int i = 0; i < m; ++i) {
+            double* vi = vv + static_cast<long>(i) * n;
+            double* vi1 = vv + static_cast<long>(i + 1) * n;
+            double* zi = z + static_cast<long>(i) * n;
+            pilu::k_step_gate<<<1, 1, 0, s>>>(dctl, i);
+            t_begin(1);
+            pilu::ilu_apply(ie, vi, zi, step_gate);
+            t_end();
+            t_begin(0);
+            spmv(lanes, s, n, rp, ci, val, 1.0, zi, 0.0, nullptr, vi1, vv, false, part0, step_gate);
+            t_end();
+            double* pin = part0;
+            double* pout = part1;
+            for (int j = 0; j <= i; ++j) {
+                const double* vnext = j < i ? vv + static_cast<long>(j + 1) * n : nullptr;
+                pilu::k_mgs<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(
+                    n, vv + static_cast<long>(j) * n, vi1, vnext, pin, pout, hh + static_cast<long>(i) * im1 + j,
+                    step_gate);
+                std::swap(pin, pout);
+            }
+            pilu::k_arnoldi_finish<<<1, pilu::kVecThreads, 0, s>>>(dctl, pin, hh, e->hist.get());
+            pilu::k_scale<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(n, vi1, dctl, true);
+            PILU_HIP(hipMemcpyAsync(&e->snap[i & 1], dctl, sizeof(h), hipMemcpyDeviceToHost, s));
+            PILU_HIP(hipEventRecord(e->snap_ev[i & 1], s));
+            if (i >= 1) {
+                PILU_HIP(hipEventSynchronize(e->snap_ev[(i - 1) & 1]));
+                if (!e->snap[(i - 1) & 1].cycle_active) break;
+            }
+        }
+        pilu::k_backsolve<<<1, 1, 0, s>>>(dctl, hh);
+        pilu::k_update<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(n, z, dctl, hh, sol);
+        PILU_HIP(hipGetLastError());
+        PILU_HIP(hipMemcpyAsync(&h, dctl, sizeof(h), hipMemcpyDeviceToHost, s));
+        PILU_HIP(hipStreamSynchronize(s));
+        its = h.its;
+        if (h.status == 2) { ok = false; break; }          // src/gmres.cpp:259-262
+        if (h.beta <= h.tol) break;                          // :278-280
+        if (its > prm.max_iterations) { ok = false; break; } // :281-283
+    }
+    its = h.its;
+    // FP64 true residual of the returned iterate (reported, not used).
+    spmv(lanes, s, n, rp, ci, val, -1.0, sol, 1.0, rhs, e->scratch.get(), nullptr, true, part0, nullptr);
+    pilu::k_sqrt_sum<<<1, pilu::kVecThreads, 0, s>>>(part0, part1);
+    PILU_HIP(hipMemcpyAsync(&e->true_res, part1, sizeof(double), hipMemcpyDeviceToHost, s));
+    pilu::check_sweep_errors(ie);
+    st.total_fgmr_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    for (const auto& p : timed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            if (p.kind == 0) st.total_mv_time += ms * 1e-3;
+            else st.total_precon_time += ms * 1e-3;
+        }
+    }
+    e->hist_len = std::min(its, prm.max_iterations + 2);
+    return {ok, its};
+}
+
+}  // namespace
+
+PreconditionedGmres::~PreconditionedGmres() { delete ext_; }
+
+std::pair<bool, int> PreconditionedGmres::Solve(const CsrMatrix* mat, double* rhs, double* sol) {
+    if (!ext_) ext_ = new Ext;
+    Ext* e = ext_;
+    const int n = mat->size;
+    const long nnz = mat->row_ptr[n];
+    const auto t0 = std::chrono::steady_clock::now();
+    e->arp.reserve(static_cast<size_t>(n) + 1);
+    e->aci.reserve(std::max<long>(nnz, 1));
+    e->aval.reserve(std::max<long>(nnz, 1));
+    e->rhs.reserve(n);
+    e->sol.reserve(n);
+    PILU_HIP(hipMemcpy(e->arp.get(), mat->row_ptr, sizeof(int) * (static_cast<size_t>(n) + 1), hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(e->aci.get(), mat->col_idx, sizeof(int) * nnz, hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(e->aval.get(), mat->value, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(e->rhs.get(), rhs, sizeof(double) * n, hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(e->sol.get(), sol, sizeof(double) * n, hipMemcpyHostToDevice));
+    stat_.total_init_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    auto r = solve_core(e, ilu_, param_, stat_, n, nnz, e->arp.get(), e->aci.get(), e->aval.get(), e->rhs.get(),
+                        e->sol.get());
+    PILU_HIP(hipMemcpy(sol, e->sol.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
+    return r;
+}
+
+std::pair<bool, int> PreconditionedGmres::SolveDevice(const CsrMatrix* d_mat, const double* d_rhs, double* d_sol) {
+    if (!ext_) ext_ = new Ext;
+    const int n = d_mat->size;
+    int h_last = 0;
+    PILU_HIP(hipMemcpy(&h_last, d_mat->row_ptr + n, sizeof(int), hipMemcpyDeviceToHost));
+    const long nnz = h_last;
+    return solve_core(ext_, ilu_, param_, stat_, n, nnz, d_mat->row_ptr, d_mat->col_idx, d_mat->value, d_rhs,
+                      d_sol);
+}
+
+double PreconditionedGmres::LastTrueResidual() const { return ext_ ? ext_->true_res : -1.0; }
+
+int PreconditionedGmres::ResidualHistory(double* out, int cap) const {
+    if (!ext_ || ext_->hist_len <= 0) return 0;
+    const int k = std::min(cap, ext_->hist_len);
+    if (k > 0) PILU_HIP(hipMemcpy(out, ext_->hist.get(), sizeof(double) * k, hipMemcpyDeviceToHost));
+    return k;
+}
+
+}  // namespace lljbash

@@ -1,0 +1,56 @@
+// Device state behind lljbash::IluSolver (private to the library).
+#pragma once
+
+#include "common.hpp"
+#include "ilu_solver.hpp"
+
+namespace pilu {
+
+// Launch-shape choice for one sweep direction.
+struct SweepShape {
+    int lanes = 8;  // lanes per row (power of two, 1..64)
+};
+
+// Apply M^-1 to device vectors on `stream`, optionally gated by a device
+// flag (kernels skip their work while *gate == 0).  Exposed for the GMRES
+// solver, which runs on the same stream.
+void ilu_apply(lljbash::IluSolver::Ext* ext, const double* d_b, double* d_x, const int* gate);
+
+hipStream_t ilu_stream(const lljbash::IluSolver::Ext* ext);
+
+// Throws if a sweep since the last check hit its watchdog.
+void check_sweep_errors(lljbash::IluSolver::Ext* ext);
+
+// Lanes-per-row for a given average row length (the sweep and SpMV
+// heuristic; the autotuner may override it).
+int lanes_for_avg(double avg_len);
+
+}  // namespace pilu
+
+struct lljbash::IluSolver::Ext {
+    int n = 0;
+    long nnz = 0;
+    int max_row = 0;
+    hipStream_t stream = nullptr;
+
+    pilu::DevBuf<int> rp, ci, dg;
+    pilu::DevBuf<int> perm_l, perm_u, level;
+    int nlev_l = 0, nlev_u = 0;
+
+    pilu::DevBuf<double> vin;    // values to factorize (scratch)
+    pilu::DevBuf<double> fac;    // factors, FP64
+    pilu::DevBuf<float> fac32;   // factors, FP32 copy (precision 32)
+    pilu::DevBuf<double> y;      // forward-sweep output, kept all-kPending between applies
+    pilu::DevBuf<double> hb;     // staging for the host Substitute path
+    pilu::DevBuf<int> ctr;       // ticket counters (pairs)
+    pilu::DevBuf<int> err;       // error words
+    pilu::DevBuf<char> sort_tmp; // radix-sort scratch
+    pilu::DevBuf<int> nzmap;     // A-entry -> F-entry map (SymbolicDevice)
+    pilu::DevBuf<signed char> fill_level;
+    long nnz_a = 0;
+
+    int precision = 64;
+    int lanes_l = 0, lanes_u = 0;  // 0 = heuristic
+    bool factored = false;
+    bool y_armed = false;
+};

@@ -1,0 +1,332 @@
+// Device kernels of the ILU preconditioner: level analysis, numeric
+// factorization and the L/U triangular sweeps.
+//
+// Scheduling.  The reference runs these loops on OpenMP threads over an
+// elimination-forest partition with per-row busy-wait flags
+// (src/ilu_solver.cpp:375-526, src/subtree.hpp).  Here every row is owned by
+// a group of G lanes (G = 64 / rows-per-wave); rows are handed out in level
+// order through per-workgroup tickets, and a row waits only for the exact
+// values it reads: each published value is its own readiness flag
+// (pilu::publish / pilu::peek_bits).  A wave that owns several rows keeps
+// polling all of them in one loop, so rows of one wave may depend on each
+// other without deadlock.
+#pragma once
+
+#include "common.hpp"
+
+namespace pilu {
+
+constexpr int kBlock = 256;          // threads per workgroup for ticketed kernels
+constexpr int kWavesPerBlock = kBlock / 64;
+
+// ---------------------------------------------------------------------
+// Diagonal pointer (src/ilu_solver.cpp:276-289) plus a missing-diagonal
+// check: dg[i] = first entry with column >= i; bad[0] counts rows whose
+// first such entry is not column i.
+// ---------------------------------------------------------------------
+__global__ void k_diag_ptr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           int* __restrict__ dg, int* __restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = rp[i], e = rp[i + 1];
+    int lo = b, hi = e;  // first position with ci >= i (rows are sorted)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ci[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    dg[i] = lo;
+    if (lo >= e || ci[lo] != i) atomicAdd(bad, 1);
+}
+
+// Row pointers: rp[0] == 0, non-decreasing, rp[n] == nnz.  Must pass
+// before any kernel reads col_idx through rp.
+__global__ void k_check_rowptr(int n, long nnz, const int* __restrict__ rp, int* __restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const bool ok = (i == 0) ? rp[0] == 0 : (rp[i] >= rp[i - 1] && (i < n || rp[n] == nnz));
+    if (!ok) atomicAdd(bad, 1);
+}
+
+// Columns in [0, n) and strictly increasing within each row.
+__global__ void k_check_sorted(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               int* __restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        const int c = ci[p];
+        if (c < 0 || c >= n || (p > rp[i] && c <= ci[p - 1])) { atomicAdd(bad, 1); return; }
+    }
+}
+
+__global__ void k_fill_int(long n, int* __restrict__ p, int v) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u64(long n, uint64_t* __restrict__ p, uint64_t v) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_iota(int n, int* __restrict__ p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// ---------------------------------------------------------------------
+// Level of each row in the L graph (row i needs rows of L(i)) or the U
+// graph (row i needs rows of U(i) \ {i}): level = 1 + max level of its
+// dependencies, 0 without any.  One lane per row; rows are visited in
+// natural order for L and reverse order for U, which is a topological
+// order, so a lane only waits on rows owned by earlier tickets or by its
+// own wave.  level[] must be -1 on entry.  maxlev[0] receives the deepest
+// level.
+// ---------------------------------------------------------------------
+template <bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict__ rp,
+                                                   const int* __restrict__ ci,
+                                                   const int* __restrict__ dg, int* level,
+                                                   int* maxlev, int* ctr, int* err, uint64_t wd_limit) {
+    const int ticket = take_ticket(ctr);
+    const Watchdog wd(wd_limit);
+    const int k = ticket * kBlock + threadIdx.x;
+    const bool active = k < n;
+    const int row = active ? (UPPER ? n - 1 - k : k) : 0;
+    int e = 0, end = 0;
+    if (active) {
+        e = UPPER ? dg[row] + 1 : rp[row];
+        end = UPPER ? rp[row + 1] : dg[row];
+    }
+    int lv = 0;
+    bool done = !active;
+    while (true) {
+        if (!done) {
+            while (e < end) {
+                const int d = peek_int(level + ci[e]);
+                if (d < 0) break;
+                lv = d + 1 > lv ? d + 1 : lv;
+                ++e;
+            }
+            if (e < end && wd.expired()) {
+                atomicOr(err, kErrTimeout);
+                e = end;
+            }
+            if (e >= end) {
+                publish_int(level + row, lv);
+                atomicMax(maxlev, lv);
+                done = true;
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        spin_pause();
+    }
+    retire_ticket(ctr);
+}
+
+// ---------------------------------------------------------------------
+// Triangular sweep.  UPPER == false: forward sweep with unit L,
+//   out[i] = b[i] - sum_{j in L(i)} l_ij out[j]          (src/ilu_solver.cpp:783-788)
+// UPPER == true: backward sweep,
+//   out[i] = (b[i] - sum_{j in U(i), j > i} u_ij out[j]) / u_ii   (:789-795)
+// `out` must hold kPending in every slot on entry.  Each row also stores
+// kPending into reset[row] after reading b[row]: the forward sweep uses it
+// to prepare the backward sweep's output, the backward sweep to re-arm its
+// input vector for the next forward sweep.  b may alias reset.
+// Lane t of a group accumulates entries t, t+G, ... in order (lane 0 starts
+// from b[i]); the group total is a fixed butterfly, so results are
+// deterministic, and bit-identical to the one-thread reference order for
+// G == 1.
+// gate (optional): the kernel does nothing when *gate == 0.
+// ---------------------------------------------------------------------
+template <int G, typename T, bool UPPER>
+__global__ __launch_bounds__(kBlock) void k_sweep(int n, const int* __restrict__ perm,
+                                                  const int* __restrict__ rp,
+                                                  const int* __restrict__ ci,
+                                                  const int* __restrict__ dg,
+                                                  const T* __restrict__ val, const double* b,
+                                                  double* out, double* reset,
+                                                  const int* gate, int* ctr, int* err,
+                                                  uint64_t wd_limit) {
+    if (gate && *gate == 0) return;
+    constexpr int R = 64 / G;
+    const int ticket = take_ticket(ctr);
+    const Watchdog wd(wd_limit);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int slot = lane / G;
+    const int t = lane & (G - 1);
+    const long pos = (static_cast<long>(ticket) * kWavesPerBlock + wave) * R + slot;
+    const bool active = pos < n;
+    int row = 0, e = 0, end = 0;
+    double acc = 0.0, dval = 1.0;
+    if (active) {
+        row = perm[pos];
+        const int d = dg[row];
+        e = (UPPER ? d + 1 : rp[row]) + t;
+        end = UPPER ? rp[row + 1] : d;
+        if (UPPER) dval = static_cast<double>(val[d]);
+        if (t == 0) {
+            acc = b[row];
+            if (reset) reinterpret_cast<uint64_t*>(reset)[row] = kPending;
+        }
+    }
+    bool done = !active;
+    while (true) {
+        if (!done) {
+            while (e < end) {
+                const uint64_t bits = peek_bits(out + ci[e]);
+                if (bits == kPending) break;
+                acc = fma(-__longlong_as_double(bits), static_cast<double>(val[e]), acc);
+                e += G;
+            }
+            if (e < end && wd.expired()) {
+                atomicOr(err, kErrTimeout);
+                acc = __longlong_as_double(kQuietNan);
+                e = end;
+            }
+        }
+        const bool ready = group_all<G>(done || e >= end);
+        if (ready && !done) {
+            const double s = group_sum<G>(acc);
+            if (t == 0) publish(out + row, UPPER ? s / dval : s);
+            done = true;
+        }
+        if (__ballot(!done) == 0) break;
+        spin_pause();
+    }
+    retire_ticket(ctr);
+}
+
+// ---------------------------------------------------------------------
+// Numeric ILU, row IKJ form of left_looking_col (src/ilu_solver.cpp:553-589):
+//   for pivots k of row i in ascending column order:
+//     a_ik /= u_kk;  a_ij -= u_kj * a_ik  for j in U(k), j > k, j in row i
+// Updates to a row entry arrive in ascending-k order, exactly as in the
+// reference; each uses one fma, the contraction the reference's
+// -ffast-math -march=native build applies.  Row i's values are staged in
+// LDS (capacity CAP per group); entries of U(k) are matched into row i by
+// binary search over its sorted columns.  vin holds the values to factor,
+// vout (all kPending on entry) receives the factors, published entry by
+// entry as value-flags; f32 (optional) receives a float copy.
+// err bit 0: a zero pivot was divided by; bit 1: a row exceeded CAP (its
+// factors are published as NaN so that dependent rows still finish).
+// ---------------------------------------------------------------------
+template <int G, int CAP>
+__global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict__ perm,
+                                                   const int* __restrict__ rp,
+                                                   const int* __restrict__ ci,
+                                                   const int* __restrict__ dg,
+                                                   const double* __restrict__ vin, double* vout,
+                                                   float* f32, int* err, int* ctr, uint64_t wd_limit) {
+    constexpr int R = 64 / G;
+    constexpr int GROUPS = kBlock / G;
+    __shared__ int s_col[GROUPS][CAP];
+    __shared__ double s_val[GROUPS][CAP];
+    const int ticket = take_ticket(ctr);
+    const Watchdog wd(wd_limit);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int slot = lane / G;
+    const int grp = wave * R + slot;
+    const int t = lane & (G - 1);
+    const long pos = static_cast<long>(ticket) * GROUPS + grp;
+    const bool active = pos < n;
+    int* col = s_col[grp];
+    double* val = s_val[grp];
+    int row = 0, rb = 0, len = 0, ndiag = 0;
+    bool overflow = false;
+    if (active) {
+        row = perm[pos];
+        rb = rp[row];
+        len = rp[row + 1] - rb;
+        ndiag = dg[row] - rb;
+        overflow = len > CAP;
+        if (!overflow) {
+            for (int q = t; q < len; q += G) {
+                col[q] = ci[rb + q];
+                val[q] = vin[rb + q];
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    int p = 0;  // next pivot (uniform within the group)
+    bool done = !active;
+    if (active && overflow) {
+        for (int q = t; q < len; q += G) {
+            publish(vout + rb + q, __longlong_as_double(kQuietNan));
+            if (f32) f32[rb + q] = __builtin_nanf("");
+        }
+        if (t == 0) atomicOr(err, 2);
+        done = true;
+    }
+    while (true) {
+        if (!done) {
+            while (p < ndiag) {
+                const int k = col[p];
+                const int dk = dg[k];
+                const int ke = rp[k + 1];
+                const uint64_t ub = peek_bits(vout + dk);
+                bool ok = ub != kPending;
+                for (int q = dk + 1 + t; ok && q < ke; q += G) ok = peek_bits(vout + q) != kPending;
+                if (!group_all<G>(ok)) break;
+                const double ukk = __longlong_as_double(ub);
+                const double lik = val[p] / ukk;
+                if (ukk == 0.0 && t == 0) atomicOr(err, 1);
+                for (int q = dk + 1 + t; q < ke; q += G) {
+                    const int j = ci[q];
+                    int lo = p + 1, hi = len;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (col[mid] < j) lo = mid + 1; else hi = mid;
+                    }
+                    if (lo < len && col[lo] == j) {
+                        const double u = __longlong_as_double(peek_bits(vout + q));
+                        val[lo] = fma(-u, lik, val[lo]);
+                    }
+                }
+                if (t == 0) val[p] = lik;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                ++p;
+            }
+            if (p < ndiag && group_all<G>(wd.expired())) {
+                if (t == 0) atomicOr(err, kErrTimeout);
+                for (int q = t; q < len; q += G) val[q] = __longlong_as_double(kQuietNan);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                p = ndiag;
+            }
+            if (p >= ndiag) {
+                for (int q = t; q < len; q += G) {
+                    const double v = val[q];
+                    if (f32) f32[rb + q] = static_cast<float>(v);
+                    publish(vout + rb + q, v);
+                }
+                done = true;
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        spin_pause();
+    }
+    retire_ticket(ctr);
+}
+
+// Zero the values to factor and arm the factor output with kPending.
+__global__ void k_prepare_factor(long nnz, double* __restrict__ vin, uint64_t* __restrict__ vout,
+                                 bool zero_in) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nnz; i += (long)gridDim.x * blockDim.x) {
+        if (zero_in) vin[i] = 0.0;
+        vout[i] = kPending;
+    }
+}
+
+// F.value[nzmap[k]] = A.value[k] (test/benchmark.cpp:116-119 on the device).
+__global__ void k_scatter_values(long nnz_a, const int* __restrict__ nzmap, const double* __restrict__ a,
+                                 double* __restrict__ f) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < nnz_a; k += (long)gridDim.x * blockDim.x)
+        f[nzmap[k]] = a[k];
+}
+
+}  // namespace pilu

@@ -1,0 +1,391 @@
+// lljbash::IluSolver on the GPU: pattern setup and level analysis,
+// numeric ILU and the triangular sweeps.
+// Interface: include/ilu_solver.hpp (reference include/ilu_solver.hpp:8-30,
+// behaviour src/ilu_solver.cpp:225-831).
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "ilu_internal.hpp"
+#include "ilu_kernels.hpp"
+#include "iluk_device.hpp"
+
+namespace pilu {
+
+[[noreturn]] void hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", static_cast<int>(e),
+                  hipGetErrorString(e), what, file, line);
+    throw std::runtime_error(buf);
+}
+
+int lanes_for_avg(double avg_len) {
+    int g = 4;
+    while (g < 64 && g < avg_len) g <<= 1;
+    return g;
+}
+
+hipStream_t ilu_stream(const lljbash::IluSolver::Ext* ext) { return ext->stream; }
+
+uint64_t watchdog_ticks(double seconds) {
+    static int rate_khz = 0;
+    if (rate_khz <= 0) {
+        int dev = 0;
+        PILU_HIP(hipGetDevice(&dev));
+        PILU_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+        if (rate_khz <= 0) rate_khz = 100000;
+    }
+    return static_cast<uint64_t>(seconds * rate_khz * 1000.0);
+}
+
+// Error-word slots of IluSolver::Ext::err.
+enum ErrSlot { kErrFactor = 0, kErrSweep = 1, kErrSorted = 2, kErrDiag = 3, kErrMaxLev = 4, kErrRowPtr = 5,
+               kErrLevels = 6 };
+constexpr double kWatchdogSeconds = 10.0;
+
+void check_sweep_errors(lljbash::IluSolver::Ext* e) {
+    int v = 0;
+    PILU_HIP(hipMemcpyAsync(&v, e->err.get() + kErrSweep, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    PILU_HIP(hipStreamSynchronize(e->stream));
+    if (v & kErrTimeout) {
+        PILU_HIP(hipMemsetAsync(e->err.get() + kErrSweep, 0, sizeof(int), e->stream));
+        e->y_armed = false;
+        throw std::runtime_error("triangular sweep: dependency wait timed out (watchdog)");
+    }
+}
+
+namespace {
+
+using Ext = lljbash::IluSolver::Ext;
+
+inline int grid_for(long n, int per_block) { return std::max(1, ceil_div(n, per_block)); }
+
+int read_int(const int* d, hipStream_t s) {
+    int h = 0;
+    PILU_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+// Levels of one dependency graph and the rows sorted by (level, row).
+int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
+    const int n = e->n;
+    hipStream_t s = e->stream;
+    int* level = e->level.get();
+    int* maxlev = e->err.get() + kErrMaxLev;
+    int* ctr = e->ctr.get();
+    const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
+    k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
+    PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
+    const int grid = grid_for(n, kBlock);
+    if (upper)
+        k_levels<true><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
+                                               e->err.get() + kErrLevels, wd);
+    else
+        k_levels<false><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
+                                                e->err.get() + kErrLevels, wd);
+    PILU_HIP(hipGetLastError());
+    const int nlev = read_int(maxlev, s) + 1;
+    if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
+        throw std::runtime_error("level analysis: dependency wait timed out (watchdog)");
+
+    DevBuf<int> keys_out(n), rows(n);
+    k_iota<<<grid_for(n, 256), 256, 0, s>>>(n, rows.get());
+    perm.reserve(n);
+    unsigned end_bit = 1;
+    while (end_bit < 31 && (1u << end_bit) <= static_cast<unsigned>(nlev)) ++end_bit;
+    size_t bytes = 0;
+    PILU_HIP(rocprim::radix_sort_pairs(nullptr, bytes, level, keys_out.get(), rows.get(), perm.get(),
+                                       static_cast<unsigned>(n), 0u, end_bit, s));
+    e->sort_tmp.reserve(bytes);
+    PILU_HIP(rocprim::radix_sort_pairs(e->sort_tmp.get(), bytes, level, keys_out.get(), rows.get(),
+                                       perm.get(), static_cast<unsigned>(n), 0u, end_bit, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    return nlev;
+}
+
+// Device pattern is in e->rp / e->ci: validate, locate diagonals, schedule.
+void analyse(Ext* e, const int* host_rp) {
+    const int n = e->n;
+    hipStream_t s = e->stream;
+    e->dg.reserve(n);
+    e->level.reserve(n);
+    e->ctr.reserve(16);
+    e->err.reserve(16);
+    PILU_HIP(hipMemsetAsync(e->ctr.get(), 0, 16 * sizeof(int), s));
+    PILU_HIP(hipMemsetAsync(e->err.get(), 0, 16 * sizeof(int), s));
+    k_check_rowptr<<<grid_for(static_cast<long>(n) + 1, 256), 256, 0, s>>>(n, e->nnz, e->rp.get(),
+                                                                        e->err.get() + kErrRowPtr);
+    PILU_HIP(hipGetLastError());
+    if (read_int(e->err.get() + kErrRowPtr, s))
+        throw std::runtime_error("row_ptr must start at 0, be non-decreasing and end at nnz");
+    k_check_sorted<<<grid_for(n, 256), 256, 0, s>>>(n, e->rp.get(), e->ci.get(), e->err.get() + kErrSorted);
+    k_diag_ptr<<<grid_for(n, 256), 256, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), e->err.get() + kErrDiag);
+    PILU_HIP(hipGetLastError());
+    int flags[2] = {0, 0};
+    PILU_HIP(hipMemcpyAsync(flags, e->err.get() + kErrSorted, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    if (flags[0]) throw std::runtime_error("rows must have strictly increasing, non-negative column indices");
+    if (flags[1]) throw std::runtime_error("missing diag");
+
+    int max_row = 0;
+    if (host_rp) {
+        for (int i = 0; i < n; ++i) max_row = std::max(max_row, host_rp[i + 1] - host_rp[i]);
+    } else {
+        std::vector<int> h(static_cast<size_t>(n) + 1);
+        PILU_HIP(hipMemcpy(h.data(), e->rp.get(), h.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) max_row = std::max(max_row, h[i + 1] - h[i]);
+    }
+    e->max_row = max_row;
+
+    e->nlev_l = build_schedule(e, false, e->perm_l);
+    e->nlev_u = build_schedule(e, true, e->perm_u);
+
+    e->vin.reserve(e->nnz);
+    e->fac.reserve(e->nnz);
+    e->y.reserve(n);
+    e->y_armed = false;
+    e->factored = false;
+}
+
+template <int G, typename T, bool UPPER>
+void sweep_one(Ext* e, const T* val, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+    constexpr int rows_per_block = (64 / G) * kWavesPerBlock;
+    const int grid = grid_for(e->n, rows_per_block);
+    k_sweep<G, T, UPPER><<<grid, kBlock, 0, e->stream>>>(e->n, UPPER ? e->perm_u.get() : e->perm_l.get(),
+                                                        e->rp.get(), e->ci.get(), e->dg.get(), val, b, out,
+                                                        reset, gate, ctr, e->err.get() + kErrSweep,
+                                                        watchdog_ticks(kWatchdogSeconds));
+}
+
+template <typename T, bool UPPER>
+void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
+                    const int* gate, int* ctr) {
+    switch (lanes) {
+        case 1: sweep_one<1, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 2: sweep_one<2, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 4: sweep_one<4, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 8: sweep_one<8, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 16: sweep_one<16, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 32: sweep_one<32, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        default: sweep_one<64, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+    }
+}
+
+int effective_lanes(const Ext* e, bool upper) {
+    const int set = upper ? e->lanes_u : e->lanes_l;
+    if (set > 0) return set;
+    const double tri = e->n > 0 ? static_cast<double>(e->nnz - e->n) / (2.0 * e->n) : 1.0;
+    return lanes_for_avg(tri);
+}
+
+template <int G, int CAP>
+void factor_one(Ext* e, float* f32) {
+    constexpr int groups = kBlock / G;
+    const int grid = grid_for(e->n, groups);
+    k_factor<G, CAP><<<grid, kBlock, 0, e->stream>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
+                                                    e->dg.get(), e->vin.get(), e->fac.get(), f32,
+                                                    e->err.get() + kErrFactor, e->ctr.get() + 2,
+                                                    watchdog_ticks(kWatchdogSeconds));
+}
+
+// Runs the factorization kernel on e->vin; returns false on a zero pivot.
+bool run_factor(Ext* e) {
+    hipStream_t s = e->stream;
+    float* f32 = nullptr;
+    if (e->precision == 32) {
+        e->fac32.reserve(e->nnz);
+        f32 = e->fac32.get();
+    }
+    PILU_HIP(hipMemsetAsync(e->err.get() + kErrFactor, 0, sizeof(int), s));
+    const int mr = e->max_row;
+    if (mr <= 32) factor_one<8, 32>(e, f32);
+    else if (mr <= 64) factor_one<16, 64>(e, f32);
+    else if (mr <= 256) factor_one<32, 256>(e, f32);
+    else if (mr <= 2560) factor_one<64, 2560>(e, f32);
+    else throw std::runtime_error("row longer than 2560 entries: not supported by the device factorization");
+    PILU_HIP(hipGetLastError());
+    const int err = read_int(e->err.get() + kErrFactor, s);
+    if (err & kErrTimeout) throw std::runtime_error("device factorization: dependency wait timed out (watchdog)");
+    if (err & 2) throw std::runtime_error("device factorization: row exceeded the LDS staging capacity");
+    e->factored = true;
+    return (err & 1) == 0;
+}
+
+Ext* need_setup(Ext* e) {
+    if (!e || e->n <= 0) throw std::runtime_error("IluSolver: SetupMatrix has not been called");
+    return e;
+}
+
+}  // namespace
+
+void ilu_apply(Ext* e, const double* d_b, double* d_x, const int* gate) {
+    need_setup(e);
+    if (!e->factored) throw std::runtime_error("IluSolver: Factorize has not been called");
+    hipStream_t s = e->stream;
+    if (!e->y_armed) {
+        k_fill_u64<<<grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(e->y.get()), kPending);
+        e->y_armed = true;
+    }
+    const int gl = effective_lanes(e, false), gu = effective_lanes(e, true);
+    int* ctr = e->ctr.get() + 4;
+    if (e->precision == 32) {
+        sweep_dispatch<float, false>(gl, e, e->fac32.get(), d_b, e->y.get(), d_x, gate, ctr);
+        sweep_dispatch<float, true>(gu, e, e->fac32.get(), e->y.get(), d_x, e->y.get(), gate, ctr + 2);
+    } else {
+        sweep_dispatch<double, false>(gl, e, e->fac.get(), d_b, e->y.get(), d_x, gate, ctr);
+        sweep_dispatch<double, true>(gu, e, e->fac.get(), e->y.get(), d_x, e->y.get(), gate, ctr + 2);
+    }
+    PILU_HIP(hipGetLastError());
+}
+
+}  // namespace pilu
+
+namespace lljbash {
+
+using pilu::DevBuf;
+
+IluSolver::~IluSolver() {
+    delete ext_;
+    ext_ = nullptr;
+    DestroyCsrMatrix(&aMatrix_);
+}
+
+void IluSolver::SetupMatrix() {
+    const int n = aMatrix_.size;
+    if (n <= 0 || !aMatrix_.row_ptr) throw std::runtime_error("IluSolver: empty matrix");
+    const long nnz = aMatrix_.row_ptr[n];
+    if (aMatrix_.row_ptr[0] != 0 || nnz < 0 || nnz > aMatrix_.max_nnz)
+        throw std::runtime_error("IluSolver: row_ptr inconsistent with the matrix storage");
+    if (!ext_) ext_ = new Ext;
+    Ext* e = ext_;
+    e->n = n;
+    e->nnz = nnz;
+    e->rp.reserve(static_cast<size_t>(n) + 1);
+    e->ci.reserve(std::max<long>(nnz, 1));
+    PILU_HIP(hipMemcpy(e->rp.get(), aMatrix_.row_ptr, sizeof(int) * (static_cast<size_t>(n) + 1),
+                       hipMemcpyHostToDevice));
+    if (nnz > 0)
+        PILU_HIP(hipMemcpy(e->ci.get(), aMatrix_.col_idx, sizeof(int) * nnz, hipMemcpyHostToDevice));
+    pilu::analyse(e, aMatrix_.row_ptr);
+}
+
+void IluSolver::SetupMatrixDevice(int n, int nnz, const int* d_row_ptr, const int* d_col_idx) {
+    if (n <= 0) throw std::runtime_error("IluSolver: empty matrix");
+    if (!ext_) ext_ = new Ext;
+    Ext* e = ext_;
+    e->n = n;
+    e->nnz = nnz;
+    e->rp.reserve(static_cast<size_t>(n) + 1);
+    e->ci.reserve(std::max(nnz, 1));
+    PILU_HIP(hipMemcpyAsync(e->rp.get(), d_row_ptr, sizeof(int) * (static_cast<size_t>(n) + 1),
+                            hipMemcpyDeviceToDevice, e->stream));
+    PILU_HIP(hipMemcpyAsync(e->ci.get(), d_col_idx, sizeof(int) * static_cast<size_t>(nnz),
+                            hipMemcpyDeviceToDevice, e->stream));
+    pilu::analyse(e, nullptr);
+}
+
+bool IluSolver::Factorize() {
+    Ext* e = pilu::need_setup(ext_);
+    if (aMatrix_.size != e->n || aMatrix_.row_ptr[e->n] != e->nnz)
+        throw std::runtime_error("IluSolver: matrix changed since SetupMatrix");
+    hipStream_t s = e->stream;
+    PILU_HIP(hipMemcpyAsync(e->vin.get(), aMatrix_.value, sizeof(double) * e->nnz, hipMemcpyHostToDevice, s));
+    pilu::k_prepare_factor<<<pilu::grid_for(e->nnz, 256 * 8), 256, 0, s>>>(
+        e->nnz, e->vin.get(), reinterpret_cast<uint64_t*>(e->fac.get()), false);
+    const bool ok = pilu::run_factor(e);
+    PILU_HIP(hipMemcpyAsync(aMatrix_.value, e->fac.get(), sizeof(double) * e->nnz, hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    return ok;
+}
+
+bool IluSolver::FactorizeDevice(const double* d_values) {
+    Ext* e = pilu::need_setup(ext_);
+    hipStream_t s = e->stream;
+    PILU_HIP(hipMemcpyAsync(e->vin.get(), d_values, sizeof(double) * e->nnz, hipMemcpyDeviceToDevice, s));
+    pilu::k_prepare_factor<<<pilu::grid_for(e->nnz, 256 * 8), 256, 0, s>>>(
+        e->nnz, e->vin.get(), reinterpret_cast<uint64_t*>(e->fac.get()), false);
+    return pilu::run_factor(e);
+}
+
+int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx, int lof) {
+    if (!ext_) ext_ = new Ext;
+    Ext* e = ext_;
+    pilu::SymbolicResult res;
+    const int nnz_f = pilu::symbolic_device(n, d_row_ptr, d_col_idx, nullptr, lof, res, e->stream);
+    if (nnz_f < 0) return -1;
+    int nnz_a = 0;
+    PILU_HIP(hipMemcpy(&nnz_a, d_row_ptr + n, sizeof(int), hipMemcpyDeviceToHost));
+    e->n = n;
+    e->nnz = nnz_f;
+    e->nnz_a = nnz_a;
+    e->rp = std::move(res.rp);
+    e->ci = std::move(res.ci);
+    e->nzmap = std::move(res.nzmap);
+    e->fill_level = std::move(res.lev);
+    pilu::analyse(e, nullptr);
+    return nnz_f;
+}
+
+bool IluSolver::FactorizeDeviceFromA(const double* d_a_values) {
+    Ext* e = pilu::need_setup(ext_);
+    if (!e->nzmap.get() || e->nnz_a <= 0) throw std::runtime_error("IluSolver: no A-to-F map (call SymbolicDevice)");
+    hipStream_t s = e->stream;
+    pilu::k_prepare_factor<<<pilu::grid_for(e->nnz, 256 * 8), 256, 0, s>>>(
+        e->nnz, e->vin.get(), reinterpret_cast<uint64_t*>(e->fac.get()), true);
+    pilu::k_scatter_values<<<pilu::grid_for(e->nnz_a, 256 * 8), 256, 0, s>>>(e->nnz_a, e->nzmap.get(), d_a_values,
+                                                                             e->vin.get());
+    return pilu::run_factor(e);
+}
+
+int IluSolver::CopyFillLevels(signed char* out) const {
+    if (!ext_ || !ext_->fill_level.get()) return 0;
+    PILU_HIP(hipMemcpy(out, ext_->fill_level.get(), sizeof(signed char) * ext_->nnz, hipMemcpyDeviceToHost));
+    return static_cast<int>(ext_->nnz);
+}
+
+void IluSolver::Substitute(const double* b, double* x) {
+    Ext* e = pilu::need_setup(ext_);
+    hipStream_t s = e->stream;
+    e->hb.reserve(e->n);
+    PILU_HIP(hipMemcpyAsync(e->hb.get(), b, sizeof(double) * e->n, hipMemcpyHostToDevice, s));
+    pilu::ilu_apply(e, e->hb.get(), e->hb.get(), nullptr);
+    PILU_HIP(hipMemcpyAsync(x, e->hb.get(), sizeof(double) * e->n, hipMemcpyDeviceToHost, s));
+    pilu::check_sweep_errors(e);
+}
+
+void IluSolver::SubstituteDevice(const double* d_b, double* d_x) {
+    pilu::ilu_apply(pilu::need_setup(ext_), d_b, d_x, nullptr);
+}
+
+void IluSolver::SetFactorPrecision(int bits) {
+    if (bits != 32 && bits != 64) throw std::runtime_error("factor precision must be 32 or 64");
+    if (!ext_) ext_ = new Ext;
+    if (ext_->precision != bits) ext_->factored = false;
+    ext_->precision = bits;
+}
+
+int IluSolver::FactorPrecision() const { return ext_ ? ext_->precision : 64; }
+
+void IluSolver::SetSweepLanes(int lanes_l, int lanes_u) {
+    auto ok = [](int g) { return g == 0 || (g >= 1 && g <= 64 && (g & (g - 1)) == 0); };
+    if (!ok(lanes_l) || !ok(lanes_u)) throw std::runtime_error("sweep lanes must be 0 or a power of two <= 64");
+    if (!ext_) ext_ = new Ext;
+    ext_->lanes_l = lanes_l;
+    ext_->lanes_u = lanes_u;
+}
+
+const int* IluSolver::DeviceRowPtr() const { return ext_ ? ext_->rp.get() : nullptr; }
+const int* IluSolver::DeviceColIdx() const { return ext_ ? ext_->ci.get() : nullptr; }
+const int* IluSolver::DeviceDiagPtr() const { return ext_ ? ext_->dg.get() : nullptr; }
+const double* IluSolver::DeviceFactors() const { return ext_ ? ext_->fac.get() : nullptr; }
+int IluSolver::NumLevelsL() const { return ext_ ? ext_->nlev_l : 0; }
+int IluSolver::NumLevelsU() const { return ext_ ? ext_->nlev_u : 0; }
+int IluSolver::Size() const { return ext_ ? ext_->n : 0; }
+int IluSolver::Nonzeros() const { return ext_ ? static_cast<int>(ext_->nnz) : 0; }
+
+}  // namespace lljbash
