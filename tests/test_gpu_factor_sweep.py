@@ -1,0 +1,154 @@
+"""GPU numeric ILU and triangular sweeps against the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import par_ilu0_cpp_amd as P
+from par_ilu0_cpp_amd import generators as gen
+from tests.conftest import dominant_values, random_pattern
+
+pytestmark = pytest.mark.gpu
+
+LANES = (1, 2, 4, 8, 16, 32, 64)
+
+
+def factored(frp, fci, fva, bits=64):
+    ilu = P.IluSolver(0)
+    ilu.SetFactorPrecision(bits)
+    ilu.SetMatrix(frp, fci, fva)
+    assert ilu.SetupMatrix() == 0
+    assert ilu.Factorize() == 0
+    return ilu
+
+
+def host_factors(ilu, nnz):
+    return P._view(ilu.GetMatrix().value, nnz, np.float64).copy()
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.mark.parametrize("case", ["conv12_ilu1", "conv10_ilu2", "rand_ilu0", "rand_ilu3", "poisson_ilu0"])
+def test_factors_match_oracle(oracle, case):
+    if case.startswith("conv"):
+        s, lof = (12, 1) if case == "conv12_ilu1" else (10, 2)
+        a = gen.convdiff3d(s)
+        rp, ci, va = a.row_ptr, a.col_idx, a.value
+    elif case == "poisson_ilu0":
+        a = gen.poisson2d(64)
+        rp, ci, va, lof = a.row_ptr, a.col_idx, a.value, 0
+    else:
+        lof = 0 if case == "rand_ilu0" else 3
+        rp, ci = random_pattern(1500, 6, 80, seed=lof)
+        va = dominant_values(rp, ci, seed=lof)
+    frp, fci, _, _, fval = None, None, None, None, None
+    o = oracle.iluk_symbolic(rp, ci, va, lof)
+    frp, fci, fval = o[0], o[1], o[3]
+    ilu = factored(frp, fci, fval)
+    fac = host_factors(ilu, fci.size)
+    ofac, dg, z = oracle.ilu_factorize(frp, fci, fval)
+    assert z == 0
+    # same elimination order and the same fma per update: expect bit equality
+    assert relmax(fac, ofac) <= 1e-12
+    assert np.array_equal(fac, ofac)
+    assert np.array_equal(ilu.DeviceFactors(), fac)
+
+
+def test_long_rows_use_wide_staging(oracle):
+    # rows of ~300 entries exercise the 64-lane / large-LDS variant
+    rp, ci = random_pattern(1200, 150, 400, seed=7)
+    va = dominant_values(rp, ci, seed=7)
+    ilu = factored(rp, ci, va)
+    ofac, dg, _ = oracle.ilu_factorize(rp, ci, va)
+    assert relmax(host_factors(ilu, ci.size), ofac) <= 1e-12
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_sweeps_all_lane_shapes(oracle, bits):
+    a = gen.convdiff3d(14)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
+    frp, fci, fval = o[0], o[1], o[3]
+    ilu = factored(frp, fci, fval, bits)
+    ofac, dg, _ = oracle.ilu_factorize(frp, fci, fval)
+    rng = np.random.default_rng(bits)
+    for lanes_l in LANES:
+        lanes_u = LANES[(LANES.index(lanes_l) + 3) % len(LANES)]
+        ilu.SetSweepLanes(lanes_l, lanes_u)
+        for rep in range(2):  # the second apply checks the re-armed buffers
+            b = rng.uniform(-1, 1, a.n)
+            x = ilu.Substitute(b)
+            if bits == 64:
+                xo = oracle.substitute(frp, fci, dg, ofac, b)
+                assert relmax(x, xo) <= 1e-12 or np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+                if lanes_l == 1 and lanes_u == 1:
+                    assert np.array_equal(x, xo)
+            else:
+                xo = oracle.substitute(frp, fci, dg, ofac.astype(np.float32).astype(np.float64), b)
+                assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+
+
+def test_sweep_lane_one_bit_exact(oracle):
+    a = gen.poisson2d(50)
+    ilu = factored(a.row_ptr, a.col_idx, a.value)
+    ofac, dg, _ = oracle.ilu_factorize(a.row_ptr, a.col_idx, a.value)
+    ilu.SetSweepLanes(1, 1)
+    b = np.random.default_rng(3).uniform(-1, 1, a.n)
+    assert np.array_equal(ilu.Substitute(b), oracle.substitute(a.row_ptr, a.col_idx, dg, ofac, b))
+
+
+def test_fp32_factor_error_bound(oracle):
+    a = gen.convdiff3d(16)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
+    ilu64 = factored(o[0], o[1], o[3], 64)
+    ilu32 = factored(o[0], o[1], o[3], 32)
+    b = np.random.default_rng(1).uniform(-1, 1, a.n)
+    x64, x32 = ilu64.Substitute(b), ilu32.Substitute(b)
+    assert np.max(np.abs(x32 - x64)) <= 1e-4 * np.max(np.abs(x64))
+    assert not np.array_equal(x32, x64)
+
+
+def test_missing_diagonal_is_minus_two():
+    rp = np.array([0, 2, 3, 5], dtype=np.int32)
+    ci = np.array([0, 2, 0, 1, 2], dtype=np.int32)  # row 1 has no (1,1)
+    ilu = P.IluSolver(0)
+    ilu.SetMatrix(rp, ci, np.ones(5))
+    assert ilu.SetupMatrix() == -2
+
+
+def test_unsorted_pattern_is_rejected():
+    rp = np.array([0, 2, 4], dtype=np.int32)
+    ci = np.array([1, 0, 0, 1], dtype=np.int32)
+    ilu = P.IluSolver(0)
+    ilu.SetMatrix(rp, ci, np.ones(4))
+    assert ilu.SetupMatrix() == -1
+
+
+def test_column_out_of_range_is_rejected():
+    rp = np.array([0, 1, 3], dtype=np.int32)
+    ci = np.array([0, 1, 7], dtype=np.int32)
+    ilu = P.IluSolver(0)
+    ilu.SetMatrix(rp, ci, np.ones(3))
+    assert ilu.SetupMatrix() == -1
+
+
+def test_zero_pivot_reports_failure():
+    rp = np.array([0, 2, 4], dtype=np.int32)
+    ci = np.array([0, 1, 0, 1], dtype=np.int32)
+    ilu = P.IluSolver(0)
+    ilu.SetMatrix(rp, ci, np.array([0.0, 1.0, 1.0, 1.0]))
+    assert ilu.SetupMatrix() == 0
+    assert ilu.Factorize() == -1
+    # the reference's factors: row 1 divides by the zero pivot
+    assert np.isinf(P._view(ilu.GetMatrix().value, 4, np.float64)[2])
+
+
+def test_refactorize_new_values(oracle):
+    a = gen.poisson2d(30)
+    ilu = factored(a.row_ptr, a.col_idx, a.value)
+    va2 = a.value * np.linspace(1.0, 2.0, a.nnz)
+    P.fill_csr(ilu.GetMatrix(), a.row_ptr, a.col_idx, va2)
+    assert ilu.Factorize() == 0
+    ofac, _, _ = oracle.ilu_factorize(a.row_ptr, a.col_idx, va2)
+    assert np.array_equal(host_factors(ilu, a.nnz), ofac)
