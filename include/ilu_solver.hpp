@@ -61,6 +61,16 @@ public:
     // Lanes cooperating on one row in the L and U sweeps (1..64, power of
     // two); 0 picks a value from the average row length.
     void SetSweepLanes(int lanes_l, int lanes_u);
+    // Per-row diagonal compensation used by the next factorizations:
+    // d_omega (device, n floats, caller-owned; null turns it off).  Row i
+    // adds omega[i] times the sum of its dropped fill updates to its pivot.
+    void SetDiagonalCompensation(const float* d_omega);
+    // Times the forward and backward sweeps for every lanes-per-row shape
+    // (reps applies each, HIP events) and keeps the fastest shape for each
+    // direction.  Results do not depend on the shape beyond rounding of the
+    // row sums.  table (optional, 7 x 2): ms per sweep for lanes 1..64.
+    // Returns 100 * lanes_l + lanes_u.
+    int AutotuneSweeps(int reps, double* table);
     // Device pointers of the current pattern and factors (valid after
     // SetupMatrix / Factorize): row_ptr, col_idx, diag_ptr, factor values.
     const int* DeviceRowPtr() const;

@@ -27,6 +27,18 @@ int IluSolverSetFactorPrecisionEx(IluSolverHdl hdl, int bits);
 #define IluSolverSetSweepLanesEx LLJBASH_DECL(IluSolverSetSweepLanesEx)
 int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
 
+/* Per-row diagonal compensation for the following factorizations: device
+ * array of n floats (kept by pointer; NULL disables).  Row i adds
+ * omega[i] * (sum of the fill updates dropped from row i) to its pivot. */
+#define IluSolverSetDiagonalCompensationEx LLJBASH_DECL(IluSolverSetDiagonalCompensationEx)
+int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
+
+/* Autotune the sweep lanes per row after Factorize: every shape 1..64 is
+ * timed over `reps` applies; table (optional, 14 doubles) receives the
+ * forward and backward ms per shape.  Returns 100 * lanes_l + lanes_u. */
+#define IluSolverAutotuneEx LLJBASH_DECL(IluSolverAutotuneEx)
+int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table);
+
 /* Number of levels of the L and U dependency graphs (after setup). */
 #define IluSolverGetLevelsEx LLJBASH_DECL(IluSolverGetLevelsEx)
 int IluSolverGetLevelsEx(IluSolverHdl hdl, int* nlev_l, int* nlev_u);
@@ -73,6 +85,35 @@ double GmresLastTrueResidualEx(GmresHdl hdl);
 /* Residual estimates |rs[i+1]| after each Arnoldi step of the last solve. */
 #define GmresResidualHistoryEx LLJBASH_DECL(GmresResidualHistoryEx)
 int GmresResidualHistoryEx(GmresHdl hdl, double* out, int cap);
+
+/* ---- batched solver: many independent systems x several RHS ----
+ * The systems are passed as one block-diagonal device CSR matrix d_a
+ * (system s owns rows sys_ptr[s] .. sys_ptr[s+1]-1, sys_ptr on the host);
+ * right-hand sides and solutions are RHS-interleaved device arrays
+ * X[row * nrhs + r], nrhs in {1, 2, 4, 8}.  Every (system, RHS) pair runs
+ * the reference's GMRES(m) with tolerance rel_tol * ||b||_2 of its own RHS;
+ * one restart cycle of all pairs is one HIP graph launch. */
+#define BatchGmresHdl LLJBASH_DECL(BatchGmresHdl)
+typedef void* BatchGmresHdl;
+#define BatchGmresCreateEx LLJBASH_DECL(BatchGmresCreateEx)
+BatchGmresHdl BatchGmresCreateEx(void);
+#define BatchGmresDestroyEx LLJBASH_DECL(BatchGmresDestroyEx)
+void BatchGmresDestroyEx(BatchGmresHdl hdl);
+/* Symbolic ILU(lof) of the block-diagonal matrix plus analysis; returns nnz(F). */
+#define BatchGmresSetupEx LLJBASH_DECL(BatchGmresSetupEx)
+int BatchGmresSetupEx(BatchGmresHdl hdl, const CsrMatrix* d_a, int lof, int nsys, const int* sys_ptr, int nrhs);
+#define BatchGmresSetFactorPrecisionEx LLJBASH_DECL(BatchGmresSetFactorPrecisionEx)
+int BatchGmresSetFactorPrecisionEx(BatchGmresHdl hdl, int bits);
+/* Factorize every system from A's device values (one launch). */
+#define BatchGmresFactorizeEx LLJBASH_DECL(BatchGmresFactorizeEx)
+int BatchGmresFactorizeEx(BatchGmresHdl hdl, const double* d_avalues);
+/* Solve all pairs; d_sol holds the initial guesses.  Host outputs (length
+ * nsys * nrhs, pair index s * nrhs + r): step counts, the reference's
+ * success flag, FP64 true relative residuals.  Returns the number of
+ * restart cycles run, or -1. */
+#define BatchGmresSolveEx LLJBASH_DECL(BatchGmresSolveEx)
+int BatchGmresSolveEx(BatchGmresHdl hdl, const CsrMatrix* d_a, const double* d_rhs, double* d_sol, double rel_tol,
+                      int m, int maxits, int* its, int* converged, double* true_rel_res);
 
 /* 1 if a HIP device is usable from this process, else 0. */
 #define DeviceAvailableEx LLJBASH_DECL(DeviceAvailableEx)

@@ -157,6 +157,34 @@ int orc_ilu_factorize(int n, const int* rp, const int* ci, const int* diag,
     return zero_pivots;
 }
 
+int orc_ilu_factorize_comp(int n, const int* rp, const int* ci, const int* diag,
+                           double* val, const float* omega) {
+    double* w = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    char* in = (char*)calloc((size_t)(n > 0 ? n : 1), 1);
+    int zero_pivots = 0;
+    if (!w || !in) { free(w); free(in); return -1; }
+    for (int i = 0; i < n; ++i) {
+        double dropped = 0.0;
+        for (int p = rp[i]; p < rp[i + 1]; ++p) { w[ci[p]] = val[p]; in[ci[p]] = 1; }
+        for (int p = rp[i]; p < diag[i]; ++p) {
+            int k = ci[p];
+            double piv = val[diag[k]];
+            if (piv == 0.0) ++zero_pivots;
+            w[k] = w[k] / piv;
+            double lik = w[k];
+            for (int q = diag[k] + 1; q < rp[k + 1]; ++q) {
+                if (in[ci[q]]) w[ci[q]] = fma(-val[q], lik, w[ci[q]]);
+                else dropped = fma(-val[q], lik, dropped);
+            }
+        }
+        if (omega) w[i] = fma((double)omega[i], dropped, w[i]);
+        for (int p = rp[i]; p < rp[i + 1]; ++p) { val[p] = w[ci[p]]; in[ci[p]] = 0; }
+    }
+    free(w);
+    free(in);
+    return zero_pivots;
+}
+
 /* ------------------------------------------------------------------ */
 /* Triangular sweeps, src/ilu_solver.cpp:775-796 (one-thread path).    */
 /* ------------------------------------------------------------------ */

@@ -55,6 +55,13 @@ void orc_diag_ptr(int n, const int* rp, const int* ci, int* diag);
 int orc_ilu_factorize(int n, const int* rp, const int* ci, const int* diag,
                       double* val);
 
+/* The same factorization with per-row diagonal compensation (modified ILU
+ * with relaxation omega[i]): updates falling outside row i's pattern are
+ * summed and omega[i] times the sum is added to a_ii when row i is done.
+ * No reference counterpart (the learned-compensation extension). */
+int orc_ilu_factorize_comp(int n, const int* rp, const int* ci, const int* diag,
+                           double* val, const float* omega);
+
 /* x = U^-1 L^-1 b, one-thread path (src/ilu_solver.cpp:775-796):
  * L ascending rows / ascending entries, U descending rows / ascending
  * entries, divide by the diagonal.  b may alias x. */

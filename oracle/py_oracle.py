@@ -33,6 +33,8 @@ def _load(fast: bool = False) -> C.CDLL:
     lib.orc_diag_ptr.argtypes = [C.c_int, _I, _I, _I]
     lib.orc_ilu_factorize.restype = C.c_int
     lib.orc_ilu_factorize.argtypes = [C.c_int, _I, _I, _I, _D]
+    lib.orc_ilu_factorize_comp.restype = C.c_int
+    lib.orc_ilu_factorize_comp.argtypes = [C.c_int, _I, _I, _I, _D, C.POINTER(C.c_float)]
     lib.orc_substitute.argtypes = [C.c_int, _I, _I, _I, _D, _D, _D]
     lib.orc_spmv.argtypes = [C.c_int, _I, _I, _D, C.c_double, _D, C.c_double, _D]
     lib.orc_gmres.restype = C.c_int
@@ -101,6 +103,17 @@ def ilu_factorize(row_ptr, col_idx, value, fast: bool = False):
     dg = diag_ptr(rp, ci)
     z = lib.orc_ilu_factorize(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(val))
     return val, dg, z
+
+
+def ilu_factorize_comp(row_ptr, col_idx, value, omega, fast: bool = False):
+    """Factorization with per-row diagonal compensation -> (factors, diag)."""
+    lib = _load(fast)
+    rp, ci = _i(row_ptr), _i(col_idx)
+    val = _d(value).copy()
+    dg = diag_ptr(rp, ci)
+    om = np.ascontiguousarray(omega, dtype=np.float32)
+    lib.orc_ilu_factorize_comp(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(val), om.ctypes.data_as(C.POINTER(C.c_float)))
+    return val, dg
 
 
 def substitute(row_ptr, col_idx, diag, factors, b, fast: bool = False):

@@ -23,6 +23,8 @@ __all__ = [
     "HostCsr",
     "IluSolver",
     "PreconditionedGmres",
+    "BatchGmres",
+    "block_diagonal",
     "IlukSymbolic",
     "iluk_symbolic_levels",
     "device_available",
@@ -173,6 +175,19 @@ class IluSolver:
         if fn("IluSolverSetSweepLanesEx")(self.hdl, int(lanes_l), int(lanes_u)) != 0:
             raise RuntimeError("bad lanes")
 
+    def SetDiagonalCompensation(self, d_omega: int) -> None:
+        """Device pointer to n float32 relaxation factors (0 disables)."""
+        if fn("IluSolverSetDiagonalCompensationEx")(self.hdl, C.c_void_p(d_omega or None)) != 0:
+            raise RuntimeError("SetDiagonalCompensation failed")
+
+    def Autotune(self, reps: int = 3):
+        """-> (lanes_l, lanes_u, table[7, 2] of ms per sweep for lanes 1..64)."""
+        table = np.zeros(14, dtype=np.float64)
+        rc = fn("IluSolverAutotuneEx")(self.hdl, int(reps), _ptr(table, C.c_double))
+        if rc < 0:
+            raise RuntimeError("IluSolverAutotuneEx failed")
+        return rc // 100, rc % 100, table.reshape(7, 2)
+
     def Levels(self) -> tuple[int, int]:
         a, b = C.c_int(0), C.c_int(0)
         fn("IluSolverGetLevelsEx")(self.hdl, C.byref(a), C.byref(b))
@@ -264,6 +279,81 @@ class PreconditionedGmres:
             self.close()
         except Exception:
             pass
+
+
+def _dev_csr(n: int, d_row_ptr: int, d_col_idx: int, d_value: int = 0) -> CsrMatrix:
+    return CsrMatrix(n, 0, C.cast(C.c_void_p(d_row_ptr), C.POINTER(C.c_int)),
+                     C.cast(C.c_void_p(d_col_idx), C.POINTER(C.c_int)),
+                     C.cast(C.c_void_p(d_value), C.POINTER(C.c_double)) if d_value else None)
+
+
+class BatchGmres:
+    """Batched ILU(k)-GMRES over a block-diagonal device matrix
+    (LLJBASH_BatchGmres*Ex): every (system, RHS) pair runs the reference
+    iteration with its own tolerance rel_tol * ||b||."""
+
+    def __init__(self):
+        self.hdl = fn("BatchGmresCreateEx")()
+        self.nsys = 0
+        self.nrhs = 0
+
+    def Setup(self, n: int, d_row_ptr: int, d_col_idx: int, lof: int, sys_ptr, nrhs: int) -> int:
+        sp = np.ascontiguousarray(sys_ptr, dtype=np.int32)
+        self.nsys, self.nrhs, self.n = sp.size - 1, int(nrhs), int(n)
+        m = _dev_csr(n, d_row_ptr, d_col_idx)
+        return int(fn("BatchGmresSetupEx")(self.hdl, C.byref(m), int(lof), self.nsys, _ptr(sp, C.c_int), self.nrhs))
+
+    def SetFactorPrecision(self, bits: int) -> None:
+        if fn("BatchGmresSetFactorPrecisionEx")(self.hdl, int(bits)) != 0:
+            raise RuntimeError("bad precision")
+
+    def Factorize(self, d_avalues: int) -> int:
+        return int(fn("BatchGmresFactorizeEx")(self.hdl, C.c_void_p(d_avalues)))
+
+    def Solve(self, d_row_ptr: int, d_col_idx: int, d_value: int, d_rhs: int, d_sol: int, rel_tol: float, m: int,
+              maxits: int):
+        """-> (cycles, its[nsys, nrhs], converged[nsys, nrhs], true_rel_res[nsys, nrhs])."""
+        ni = self.nsys * self.nrhs
+        its = np.zeros(ni, dtype=np.int32)
+        conv = np.zeros(ni, dtype=np.int32)
+        rel = np.zeros(ni, dtype=np.float64)
+        mat = _dev_csr(self.n, d_row_ptr, d_col_idx, d_value)
+        cyc = fn("BatchGmresSolveEx")(self.hdl, C.byref(mat), C.c_void_p(d_rhs), C.c_void_p(d_sol), float(rel_tol),
+                                      int(m), int(maxits), _ptr(its, C.c_int), _ptr(conv, C.c_int),
+                                      _ptr(rel, C.c_double))
+        if cyc < 0:
+            raise RuntimeError("BatchGmresSolveEx failed")
+        shape = (self.nsys, self.nrhs)
+        return int(cyc), its.reshape(shape), conv.reshape(shape).astype(bool), rel.reshape(shape)
+
+    def close(self) -> None:
+        if self.hdl:
+            fn("BatchGmresDestroyEx")(self.hdl)
+            self.hdl = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def block_diagonal(mats):
+    """Assemble host CSR systems into one block-diagonal CSR; returns
+    (row_ptr, col_idx, value, sys_ptr)."""
+    sys_ptr = np.zeros(len(mats) + 1, dtype=np.int64)
+    sys_ptr[1:] = np.cumsum([m.n for m in mats])
+    nnz_ptr = np.zeros(len(mats) + 1, dtype=np.int64)
+    nnz_ptr[1:] = np.cumsum([m.nnz for m in mats])
+    rp = np.empty(sys_ptr[-1] + 1, dtype=np.int64)
+    rp[-1] = nnz_ptr[-1]
+    ci = np.empty(nnz_ptr[-1], dtype=np.int32)
+    va = np.empty(nnz_ptr[-1], dtype=np.float64)
+    for k, m in enumerate(mats):
+        rp[sys_ptr[k]:sys_ptr[k + 1]] = m.row_ptr[:-1] + nnz_ptr[k]
+        ci[nnz_ptr[k]:nnz_ptr[k + 1]] = m.col_idx + sys_ptr[k]
+        va[nnz_ptr[k]:nnz_ptr[k + 1]] = m.value
+    return rp.astype(np.int32), ci, va, sys_ptr.astype(np.int32)
 
 
 def device_available() -> bool:

@@ -85,6 +85,8 @@ SIGNATURES = [
     ("IluSolverSetFactorPrecisionEx", _int, [_vp, _int]),
     ("IluSolverSetSweepLanesEx", _int, [_vp, _int, _int]),
     ("IluSolverGetLevelsEx", _int, [_vp, P(_int), P(_int)]),
+    ("IluSolverAutotuneEx", _int, [_vp, _int, P(_dbl)]),
+    ("IluSolverSetDiagonalCompensationEx", _int, [_vp, _vp]),
     ("IluSolverSymbolicDeviceEx", _int, [_vp, P(CsrMatrix), _int]),
     ("IluSolverFactorizeDeviceEx", _int, [_vp, _vp, _int]),
     ("IluSolverSubstituteDeviceEx", _int, [_vp, _vp, _vp]),
@@ -96,6 +98,12 @@ SIGNATURES = [
     ("GmresLastTrueResidualEx", _dbl, [_vp]),
     ("GmresResidualHistoryEx", _int, [_vp, P(_dbl), _int]),
     ("DeviceAvailableEx", _int, []),
+    ("BatchGmresCreateEx", _vp, []),
+    ("BatchGmresDestroyEx", None, [_vp]),
+    ("BatchGmresSetupEx", _int, [_vp, P(CsrMatrix), _int, _int, P(_int), _int]),
+    ("BatchGmresSetFactorPrecisionEx", _int, [_vp, _int]),
+    ("BatchGmresFactorizeEx", _int, [_vp, _vp]),
+    ("BatchGmresSolveEx", _int, [_vp, P(CsrMatrix), _vp, _vp, _dbl, _int, _int, P(_int), P(_int), P(_dbl)]),
 ]
 
 DATA_SYMBOLS = ["CSR_MATRIX_DEFAULT"]

@@ -40,6 +40,14 @@ int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u) {
     return guarded("SetSweepLanes", [&] { S(hdl)->SetSweepLanes(lanes_l, lanes_u); return 0; });
 }
 
+int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega) {
+    return guarded("SetDiagonalCompensation", [&] { S(hdl)->SetDiagonalCompensation(d_omega); return 0; });
+}
+
+int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table) {
+    return guarded("Autotune", [&] { return S(hdl)->AutotuneSweeps(reps, table); });
+}
+
 int IluSolverGetLevelsEx(IluSolverHdl hdl, int* nlev_l, int* nlev_u) {
     if (nlev_l) *nlev_l = S(hdl)->NumLevelsL();
     if (nlev_u) *nlev_u = S(hdl)->NumLevelsU();

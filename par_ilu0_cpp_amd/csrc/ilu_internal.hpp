@@ -48,6 +48,7 @@ struct lljbash::IluSolver::Ext {
     pilu::DevBuf<int> nzmap;     // A-entry -> F-entry map (SymbolicDevice)
     pilu::DevBuf<signed char> fill_level;
     long nnz_a = 0;
+    const float* omega_ptr = nullptr;  // per-row compensation (device, caller-owned)
 
     int precision = 64;
     int lanes_l = 0, lanes_u = 0;  // 0 = heuristic

@@ -208,6 +208,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, const int* __restrict__
 // entry as value-flags; f32 (optional) receives a float copy.
 // err bit 0: a zero pivot was divided by; bit 1: a row exceeded CAP (its
 // factors are published as NaN so that dependent rows still finish).
+// omega (optional): per-row diagonal compensation.  Updates u_kj * a_ik that
+// fall outside row i's pattern are summed (lane-strided, then a fixed
+// butterfly) and omega[i] times that sum is added to a_ii before the row is
+// published -- modified ILU with a per-row relaxation factor.
 // ---------------------------------------------------------------------
 template <int G, int CAP>
 __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict__ perm,
@@ -215,7 +219,8 @@ __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict_
                                                    const int* __restrict__ ci,
                                                    const int* __restrict__ dg,
                                                    const double* __restrict__ vin, double* vout,
-                                                   float* f32, int* err, int* ctr, uint64_t wd_limit) {
+                                                   float* f32, int* err, int* ctr, uint64_t wd_limit,
+                                                   const float* __restrict__ omega) {
     constexpr int R = 64 / G;
     constexpr int GROUPS = kBlock / G;
     __shared__ int s_col[GROUPS][CAP];
@@ -250,6 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict_
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     int p = 0;  // next pivot (uniform within the group)
+    double dropped = 0.0;  // this lane's share of the fill updates outside the pattern
     bool done = !active;
     if (active && overflow) {
         for (int q = t; q < len; q += G) {
@@ -279,10 +285,9 @@ __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict_
                         const int mid = (lo + hi) >> 1;
                         if (col[mid] < j) lo = mid + 1; else hi = mid;
                     }
-                    if (lo < len && col[lo] == j) {
-                        const double u = __longlong_as_double(peek_bits(vout + q));
-                        val[lo] = fma(-u, lik, val[lo]);
-                    }
+                    const double u = __longlong_as_double(peek_bits(vout + q));
+                    if (lo < len && col[lo] == j) val[lo] = fma(-u, lik, val[lo]);
+                    else dropped = fma(-u, lik, dropped);
                 }
                 if (t == 0) val[p] = lik;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -299,6 +304,14 @@ __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict_
                 p = ndiag;
             }
             if (p >= ndiag) {
+                if (omega) {
+                    // diagonal compensation: u_ii += omega_i * (sum of dropped fill)
+                    const double dsum = group_sum<G>(dropped);
+                    if (t == 0) val[ndiag] = fma(static_cast<double>(omega[row]), dsum, val[ndiag]);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
                 for (int q = t; q < len; q += G) {
                     const double v = val[q];
                     if (f32) f32[rb + q] = static_cast<float>(v);

@@ -190,7 +190,7 @@ void factor_one(Ext* e, float* f32) {
     k_factor<G, CAP><<<grid, kBlock, 0, e->stream>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
                                                     e->dg.get(), e->vin.get(), e->fac.get(), f32,
                                                     e->err.get() + kErrFactor, e->ctr.get() + 2,
-                                                    watchdog_ticks(kWatchdogSeconds));
+                                                    watchdog_ticks(kWatchdogSeconds), e->omega_ptr);
 }
 
 // Runs the factorization kernel on e->vin; returns false on a zero pivot.
@@ -360,6 +360,65 @@ void IluSolver::Substitute(const double* b, double* x) {
 
 void IluSolver::SubstituteDevice(const double* d_b, double* d_x) {
     pilu::ilu_apply(pilu::need_setup(ext_), d_b, d_x, nullptr);
+}
+
+void IluSolver::SetDiagonalCompensation(const float* d_omega) {
+    if (!ext_) ext_ = new Ext;
+    ext_->omega_ptr = d_omega;
+}
+
+int IluSolver::AutotuneSweeps(int reps, double* table) {
+    Ext* e = pilu::need_setup(ext_);
+    if (!e->factored) throw std::runtime_error("IluSolver: Factorize before AutotuneSweeps");
+    hipStream_t s = e->stream;
+    static const int kLanes[7] = {1, 2, 4, 8, 16, 32, 64};
+    pilu::DevBuf<double> b(e->n), x(e->n);
+    pilu::k_fill_u64<<<pilu::grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(b.get()),
+                                                                   0x3FF0000000000000ull);  // 1.0
+    hipEvent_t ev[3];
+    for (auto& v : ev) PILU_HIP(hipEventCreate(&v));
+    const int save_l = e->lanes_l, save_u = e->lanes_u;
+    double best[2] = {1e300, 1e300};
+    int pick[2] = {8, 8};
+    for (int k = 0; k < 7; ++k) {
+        e->lanes_l = e->lanes_u = kLanes[k];
+        pilu::ilu_apply(e, b.get(), x.get(), nullptr);  // warm
+        double tl = 0.0, tu = 0.0;
+        for (int r = 0; r < std::max(reps, 1); ++r) {
+            int* ctr = e->ctr.get() + 4;
+            PILU_HIP(hipEventRecord(ev[0], s));
+            if (e->precision == 32)
+                pilu::sweep_dispatch<float, false>(kLanes[k], e, e->fac32.get(), b.get(), e->y.get(), x.get(), nullptr, ctr);
+            else
+                pilu::sweep_dispatch<double, false>(kLanes[k], e, e->fac.get(), b.get(), e->y.get(), x.get(), nullptr, ctr);
+            PILU_HIP(hipEventRecord(ev[1], s));
+            if (e->precision == 32)
+                pilu::sweep_dispatch<float, true>(kLanes[k], e, e->fac32.get(), e->y.get(), x.get(), e->y.get(), nullptr,
+                                                  ctr + 2);
+            else
+                pilu::sweep_dispatch<double, true>(kLanes[k], e, e->fac.get(), e->y.get(), x.get(), e->y.get(), nullptr,
+                                                   ctr + 2);
+            PILU_HIP(hipEventRecord(ev[2], s));
+            PILU_HIP(hipEventSynchronize(ev[2]));
+            float a = 0.f, c = 0.f;
+            PILU_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+            PILU_HIP(hipEventElapsedTime(&c, ev[1], ev[2]));
+            tl += a;
+            tu += c;
+        }
+        tl /= std::max(reps, 1);
+        tu /= std::max(reps, 1);
+        if (table) { table[2 * k] = tl; table[2 * k + 1] = tu; }
+        if (tl < best[0]) { best[0] = tl; pick[0] = kLanes[k]; }
+        if (tu < best[1]) { best[1] = tu; pick[1] = kLanes[k]; }
+    }
+    for (auto& v : ev) (void)hipEventDestroy(v);
+    pilu::check_sweep_errors(e);
+    (void)save_l;
+    (void)save_u;
+    e->lanes_l = pick[0];
+    e->lanes_u = pick[1];
+    return pick[0] * 100 + pick[1];
 }
 
 void IluSolver::SetFactorPrecision(int bits) {

@@ -152,3 +152,17 @@ def test_refactorize_new_values(oracle):
     assert ilu.Factorize() == 0
     ofac, _, _ = oracle.ilu_factorize(a.row_ptr, a.col_idx, va2)
     assert np.array_equal(host_factors(ilu, a.nnz), ofac)
+
+
+def test_autotune_keeps_results(oracle):
+    a = gen.convdiff3d(18)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
+    ilu = factored(o[0], o[1], o[3])
+    b = np.random.default_rng(9).uniform(-1, 1, a.n)
+    ilu.SetSweepLanes(8, 8)
+    x0 = ilu.Substitute(b)
+    gl, gu, table = ilu.Autotune(reps=2)
+    assert gl in LANES and gu in LANES and np.all(table > 0)
+    assert table[LANES.index(gl), 0] == table[:, 0].min() and table[LANES.index(gu), 1] == table[:, 1].min()
+    x1 = ilu.Substitute(b)
+    assert np.max(np.abs(x1 - x0)) <= 1e-12 * np.max(np.abs(x0))

@@ -121,3 +121,22 @@ def test_zero_pivot_is_counted(oracle):
     ci = np.array([0, 1, 0, 1], dtype=np.int32)
     _, _, z = oracle.ilu_factorize(rp, ci, np.array([0.0, 1.0, 1.0, 1.0]))
     assert z == 1
+
+
+@pytest.mark.parametrize("lof", [0, 1])
+def test_compensated_factorization_row_sums(oracle, lof):
+    """Modified ILU with omega = 1 preserves row sums: (L U) 1 = A 1; with
+    omega = 0 it is plain ILU, bit for bit."""
+    a = gen.convdiff3d(7)
+    frp, fci, _, fval, _ = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, lof)
+    n = a.n
+    fac1, dg = oracle.ilu_factorize_comp(frp, fci, fval, np.ones(n, dtype=np.float32))
+    L, U = split_lu(frp, fci, fac1, n)
+    A = dense(a.row_ptr, a.col_idx, a.value, n)
+    assert np.allclose((L @ U).sum(axis=1), A.sum(axis=1), rtol=0, atol=1e-13)
+    fac0, _ = oracle.ilu_factorize_comp(frp, fci, fval, np.zeros(n, dtype=np.float32))
+    plain, _, _ = oracle.ilu_factorize(frp, fci, fval)
+    assert np.array_equal(fac0, plain)
+    if lof == 0:
+        # the compensation actually changed something (ILU(0) drops fill here)
+        assert not np.array_equal(fac1, plain)
