@@ -115,6 +115,45 @@ int BatchGmresFactorizeEx(BatchGmresHdl hdl, const double* d_avalues);
 int BatchGmresSolveEx(BatchGmresHdl hdl, const CsrMatrix* d_a, const double* d_rhs, double* d_sol, double rel_tol,
                       int m, int maxits, int* its, int* converged, double* true_rel_res);
 
+/* ---- learned diagonal compensation (3-layer message-passing network,
+ * hidden width 32, BF16 MFMA with FP32 accumulation) ----
+ * The network reads a device matrix and predicts omega (n floats, device)
+ * for IluSolverSetDiagonalCompensationEx.  Parameters are a flat FP32
+ * vector of GnnParameterCountEx() values. */
+#define GnnHdl LLJBASH_DECL(GnnHdl)
+typedef void* GnnHdl;
+#define GnnCreateEx LLJBASH_DECL(GnnCreateEx)
+GnnHdl GnnCreateEx(void);
+#define GnnDestroyEx LLJBASH_DECL(GnnDestroyEx)
+void GnnDestroyEx(GnnHdl hdl);
+#define GnnParameterCountEx LLJBASH_DECL(GnnParameterCountEx)
+int GnnParameterCountEx(void);
+#define GnnSetParametersEx LLJBASH_DECL(GnnSetParametersEx)
+int GnnSetParametersEx(GnnHdl hdl, const float* host_params);
+#define GnnGetParametersEx LLJBASH_DECL(GnnGetParametersEx)
+int GnnGetParametersEx(GnnHdl hdl, float* host_params);
+/* omega = network(A); d_a holds device arrays (values included). */
+#define GnnPredictEx LLJBASH_DECL(GnnPredictEx)
+int GnnPredictEx(GnnHdl hdl, const CsrMatrix* d_a, float* d_omega);
+/* Gradients of the mean squared error between omega and d_target (n
+ * floats).  d_at: device CSR pattern of A^T; d_tmap[k]: index in A of the
+ * k-th entry of A^T.  host_grads (optional) receives the gradient. */
+#define GnnGradientsEx LLJBASH_DECL(GnnGradientsEx)
+int GnnGradientsEx(GnnHdl hdl, const CsrMatrix* d_a, const CsrMatrix* d_at, const int* d_tmap, const float* d_target,
+                   float* host_grads, double* loss);
+/* One Adam step (lr) on that loss. */
+#define GnnTrainStepEx LLJBASH_DECL(GnnTrainStepEx)
+int GnnTrainStepEx(GnnHdl hdl, const CsrMatrix* d_a, const CsrMatrix* d_at, const int* d_tmap, const float* d_target,
+                   float lr, double* loss);
+/* Learned solve with fallback: omega = network(A), compensated factorization
+ * of the solver's pattern (set up by IluSolverSymbolicDeviceEx) from A's
+ * values, GMRES from d_sol.  If that solve fails or needs more than
+ * `budget` steps, the plain factorization is solved again from the same
+ * initial guess and *fallback is set.  Returns the converged flag. */
+#define GmresSolveLearnedDeviceEx LLJBASH_DECL(GmresSolveLearnedDeviceEx)
+bool GmresSolveLearnedDeviceEx(GmresHdl gmres, IluSolverHdl ilu, GnnHdl gnn, const CsrMatrix* d_a,
+                               const double* d_rhs, double* d_sol, int budget, int* iter, int* fallback);
+
 /* 1 if a HIP device is usable from this process, else 0. */
 #define DeviceAvailableEx LLJBASH_DECL(DeviceAvailableEx)
 int DeviceAvailableEx(void);

@@ -98,6 +98,15 @@ SIGNATURES = [
     ("GmresLastTrueResidualEx", _dbl, [_vp]),
     ("GmresResidualHistoryEx", _int, [_vp, P(_dbl), _int]),
     ("DeviceAvailableEx", _int, []),
+    ("GnnCreateEx", _vp, []),
+    ("GnnDestroyEx", None, [_vp]),
+    ("GnnParameterCountEx", _int, []),
+    ("GnnSetParametersEx", _int, [_vp, P(C.c_float)]),
+    ("GnnGetParametersEx", _int, [_vp, P(C.c_float)]),
+    ("GnnPredictEx", _int, [_vp, P(CsrMatrix), _vp]),
+    ("GnnGradientsEx", _int, [_vp, P(CsrMatrix), P(CsrMatrix), _vp, _vp, P(C.c_float), P(_dbl)]),
+    ("GnnTrainStepEx", _int, [_vp, P(CsrMatrix), P(CsrMatrix), _vp, _vp, C.c_float, P(_dbl)]),
+    ("GmresSolveLearnedDeviceEx", C.c_bool, [_vp, _vp, _vp, P(CsrMatrix), _vp, _vp, _int, P(_int), P(_int)]),
     ("BatchGmresCreateEx", _vp, []),
     ("BatchGmresDestroyEx", None, [_vp]),
     ("BatchGmresSetupEx", _int, [_vp, P(CsrMatrix), _int, _int, P(_int), _int]),

@@ -32,7 +32,7 @@ struct GmresCtl {
     int status;        // 0 running, 1 converged at cycle start, 2 NaN estimate
     int breakdown;     // ||w|| <= 1e-16 in the last step
     int hist_cap;
-    int pad;
+    int nhist;         // residual estimates written to the history
     double tol;
     double beta;       // latest residual estimate
     double inv;        // scale for the vector just normalised
@@ -200,7 +200,7 @@ __global__ void k_arnoldi_finish(GmresCtl* ctl, const double* part, double* hh, 
     col[i] = c[i] * col[i] + s[i] * col[i1];
     const double beta = std::fabs(rs[i1]);
     ctl->beta = beta;
-    if (hist && ctl->its - 1 < ctl->hist_cap) hist[ctl->its - 1] = beta;
+    if (hist && ctl->nhist < ctl->hist_cap) hist[ctl->nhist++] = beta;
     if (std::isnan(beta)) {
         ctl->status = 2;
         ctl->cycle_active = 0;
@@ -452,7 +452,7 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
             else st.total_precon_time += ms * 1e-3;
         }
     }
-    e->hist_len = std::min(its, prm.max_iterations + 2);
+    e->hist_len = h.nhist;
     return {ok, its};
 }
 

@@ -1,0 +1,49 @@
+"""Pins the network oracle (oracle/gnn_oracle.py): analytic gradients of the
+mean-squared-error loss against central finite differences in FP64, and
+the BF16 rounding helper against exact bit patterns."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import gnn_oracle as G
+from par_ilu0_cpp_amd import generators as gen
+
+
+def small_problem(seed=0):
+    kappa = np.exp(np.random.default_rng(seed).normal(0, 1, size=(5, 4, 3)))
+    a = gen.diffusion3d(kappa)
+    feat, w = G.features(a.row_ptr, a.col_idx, a.value)
+    rng = np.random.default_rng(seed + 1)
+    p = rng.normal(0, 0.3, G.N_PARAMS)
+    t = rng.uniform(0.2, 0.9, a.n)
+    return a, feat, w, p, t
+
+
+def test_bf16_rounding_bits():
+    x = np.array([1.0, 1.00390625, 1.005859375, -3.0e-5, 65504.0], dtype=np.float32)
+    r = G.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0  # tie rounds to even
+    assert r[2] == 1.0078125
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2.0**-8)
+
+
+def test_gradients_match_finite_differences():
+    a, feat, w, p, t = small_problem()
+    loss, g = G.loss_grad(p, a.row_ptr, a.col_idx, feat, w, t, use_bf16=False)
+    rng = np.random.default_rng(3)
+    idx = np.concatenate([rng.choice(G.N_PARAMS - 33, 40, replace=False), np.arange(G.N_PARAMS - 33, G.N_PARAMS)])
+    h = 1e-6
+    for j in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[j] += h
+        pm[j] -= h
+        lp, _ = G.loss_grad(pp, a.row_ptr, a.col_idx, feat, w, t, use_bf16=False)
+        lm, _ = G.loss_grad(pm, a.row_ptr, a.col_idx, feat, w, t, use_bf16=False)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[j]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (j, fd, g[j])
+
+
+def test_features_are_scale_free():
+    a, feat, w, _, _ = small_problem(2)
+    feat2, w2 = G.features(a.row_ptr, a.col_idx, 7.5 * a.value)
+    assert np.allclose(feat, feat2, rtol=1e-6) and np.allclose(w, w2, rtol=1e-6)
