@@ -65,12 +65,19 @@ public:
     // d_omega (device, n floats, caller-owned; null turns it off).  Row i
     // adds omega[i] times the sum of its dropped fill updates to its pivot.
     void SetDiagonalCompensation(const float* d_omega);
-    // Times the forward and backward sweeps for every lanes-per-row shape
-    // (reps applies each, HIP events) and keeps the fastest shape for each
-    // direction.  Results do not depend on the shape beyond rounding of the
-    // row sums.  table (optional, 7 x 2): ms per sweep for lanes 1..64.
-    // Returns 100 * lanes_l + lanes_u.
-    int AutotuneSweeps(int reps, double* table);
+    // Times the sweeps in every launch shape -- level-scheduled rows with
+    // 1..64 lanes per row, and block-local sweeps (blocks of 4096, 8192 or
+    // 16384 rows staged in LDS) with 4..32 lanes -- over `reps` applies
+    // each, and keeps the fastest shape for each direction.  Results depend
+    // on the lanes per row only through the rounding of row sums.
+    // table (optional, cap rows of 5 doubles): mode, block rows, lanes,
+    // forward ms, backward ms.  Returns the number of shapes timed.
+    int AutotuneSweeps(int reps, double* table, int cap);
+    // Sweep algorithm per direction: 0 level-scheduled rows, 1 block-local
+    // with blocks of blk_rows rows (64..16384).
+    void SetSweepMode(int mode_l, int mode_u, int blk_rows);
+    // cfg[5]: mode_l, lanes_l, mode_u, lanes_u, blk_rows.
+    void GetSweepConfig(int* cfg) const;
     // Device pointers of the current pattern and factors (valid after
     // SetupMatrix / Factorize): row_ptr, col_idx, diag_ptr, factor values.
     const int* DeviceRowPtr() const;

@@ -33,11 +33,21 @@ int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
 #define IluSolverSetDiagonalCompensationEx LLJBASH_DECL(IluSolverSetDiagonalCompensationEx)
 int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
 
-/* Autotune the sweep lanes per row after Factorize: every shape 1..64 is
- * timed over `reps` applies; table (optional, 14 doubles) receives the
- * forward and backward ms per shape.  Returns 100 * lanes_l + lanes_u. */
+/* Autotune the sweeps after Factorize: level-scheduled rows with 1..64
+ * lanes per row and block-local sweeps (4096/8192/16384-row blocks in LDS)
+ * with 4..32 lanes are each timed over `reps` applies; the fastest shape
+ * per direction is kept.  table (optional, cap rows x 5 doubles): mode,
+ * block rows, lanes, forward ms, backward ms.  Returns the rows timed. */
 #define IluSolverAutotuneEx LLJBASH_DECL(IluSolverAutotuneEx)
-int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table);
+int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap);
+
+/* Explicit sweep algorithm per direction (0 level rows, 1 block-local with
+ * blocks of blk_rows rows) and the current configuration
+ * (cfg[5]: mode_l, lanes_l, mode_u, lanes_u, blk_rows). */
+#define IluSolverSetSweepModeEx LLJBASH_DECL(IluSolverSetSweepModeEx)
+int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_rows);
+#define IluSolverGetSweepConfigEx LLJBASH_DECL(IluSolverGetSweepConfigEx)
+int IluSolverGetSweepConfigEx(IluSolverHdl hdl, int* cfg);
 
 /* Number of levels of the L and U dependency graphs (after setup). */
 #define IluSolverGetLevelsEx LLJBASH_DECL(IluSolverGetLevelsEx)

@@ -181,12 +181,23 @@ class IluSolver:
             raise RuntimeError("SetDiagonalCompensation failed")
 
     def Autotune(self, reps: int = 3):
-        """-> (lanes_l, lanes_u, table[7, 2] of ms per sweep for lanes 1..64)."""
-        table = np.zeros(14, dtype=np.float64)
-        rc = fn("IluSolverAutotuneEx")(self.hdl, int(reps), _ptr(table, C.c_double))
+        """Times every sweep shape and keeps the fastest per direction.
+        -> (config dict, table[rows, 5]: mode, block rows, lanes, fwd ms, bwd ms)."""
+        table = np.zeros((64, 5), dtype=np.float64)
+        rc = fn("IluSolverAutotuneEx")(self.hdl, int(reps), _ptr(table, C.c_double), 64)
         if rc < 0:
             raise RuntimeError("IluSolverAutotuneEx failed")
-        return rc // 100, rc % 100, table.reshape(7, 2)
+        return self.SweepConfig(), table[:rc].copy()
+
+    def SetSweepMode(self, mode_l: int, mode_u: int, blk_rows: int = 0) -> None:
+        if fn("IluSolverSetSweepModeEx")(self.hdl, int(mode_l), int(mode_u), int(blk_rows)) != 0:
+            raise RuntimeError("SetSweepMode failed")
+
+    def SweepConfig(self) -> dict:
+        cfg = np.zeros(5, dtype=np.int32)
+        fn("IluSolverGetSweepConfigEx")(self.hdl, _ptr(cfg, C.c_int))
+        return dict(mode_l=int(cfg[0]), lanes_l=int(cfg[1]), mode_u=int(cfg[2]), lanes_u=int(cfg[3]),
+                    blk_rows=int(cfg[4]))
 
     def Levels(self) -> tuple[int, int]:
         a, b = C.c_int(0), C.c_int(0)

@@ -85,7 +85,9 @@ SIGNATURES = [
     ("IluSolverSetFactorPrecisionEx", _int, [_vp, _int]),
     ("IluSolverSetSweepLanesEx", _int, [_vp, _int, _int]),
     ("IluSolverGetLevelsEx", _int, [_vp, P(_int), P(_int)]),
-    ("IluSolverAutotuneEx", _int, [_vp, _int, P(_dbl)]),
+    ("IluSolverAutotuneEx", _int, [_vp, _int, P(_dbl), _int]),
+    ("IluSolverSetSweepModeEx", _int, [_vp, _int, _int, _int]),
+    ("IluSolverGetSweepConfigEx", _int, [_vp, P(_int)]),
     ("IluSolverSetDiagonalCompensationEx", _int, [_vp, _vp]),
     ("IluSolverSymbolicDeviceEx", _int, [_vp, P(CsrMatrix), _int]),
     ("IluSolverFactorizeDeviceEx", _int, [_vp, _vp, _int]),

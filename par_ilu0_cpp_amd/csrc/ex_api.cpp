@@ -44,8 +44,17 @@ int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega) {
     return guarded("SetDiagonalCompensation", [&] { S(hdl)->SetDiagonalCompensation(d_omega); return 0; });
 }
 
-int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table) {
-    return guarded("Autotune", [&] { return S(hdl)->AutotuneSweeps(reps, table); });
+int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap) {
+    return guarded("Autotune", [&] { return S(hdl)->AutotuneSweeps(reps, table, cap); });
+}
+
+int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_rows) {
+    return guarded("SetSweepMode", [&] { S(hdl)->SetSweepMode(mode_l, mode_u, blk_rows); return 0; });
+}
+
+int IluSolverGetSweepConfigEx(IluSolverHdl hdl, int* cfg) {
+    S(hdl)->GetSweepConfig(cfg);
+    return 0;
 }
 
 int IluSolverGetLevelsEx(IluSolverHdl hdl, int* nlev_l, int* nlev_u) {

@@ -52,6 +52,11 @@ struct lljbash::IluSolver::Ext {
 
     int precision = 64;
     int lanes_l = 0, lanes_u = 0;  // 0 = heuristic
+    // Sweep algorithm per direction: 0 = level-scheduled rows (k_sweep),
+    // 1 = block-local (k_sweep_blk) with blocks of blk_rows rows.
+    int mode_l = 0, mode_u = 0;
+    int blk_rows = 0, nblk = 0;
+    pilu::DevBuf<int> bperm_l, bperm_u, blev_l, blev_u, loff_l, loff_u;
     bool factored = false;
     bool y_armed = false;
 };

@@ -78,13 +78,15 @@ __global__ void k_iota(int n, int* __restrict__ p) {
 // natural order for L and reverse order for U, which is a topological
 // order, so a lane only waits on rows owned by earlier tickets or by its
 // own wave.  level[] must be -1 on entry.  maxlev[0] receives the deepest
-// level.
+// level.  blk > 0 restricts the graph to dependencies inside the row's block
+// of blk consecutive rows (local levels of the block sweep).
 // ---------------------------------------------------------------------
 template <bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict__ rp,
                                                    const int* __restrict__ ci,
                                                    const int* __restrict__ dg, int* level,
-                                                   int* maxlev, int* ctr, int* err, uint64_t wd_limit) {
+                                                   int* maxlev, int* ctr, int* err, uint64_t wd_limit,
+                                                   int blk) {
     const int ticket = take_ticket(ctr);
     const Watchdog wd(wd_limit);
     const int k = ticket * kBlock + threadIdx.x;
@@ -100,7 +102,9 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
     while (true) {
         if (!done) {
             while (e < end) {
-                const int d = peek_int(level + ci[e]);
+                const int c = ci[e];
+                if (blk > 0 && c / blk != row / blk) { ++e; continue; }  // outside the row's block
+                const int d = peek_int(level + c);
                 if (d < 0) break;
                 lv = d + 1 > lv ? d + 1 : lv;
                 ++e;
@@ -192,6 +196,80 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, const int* __restrict__
         if (__ballot(!done) == 0) break;
         spin_pause();
     }
+    retire_ticket(ctr);
+}
+
+// ---------------------------------------------------------------------
+// Block-local triangular sweep.  Rows are cut into blocks of `blk`
+// consecutive rows; one workgroup owns a block and keeps the block's results
+// in LDS.  Inside the block it walks the block's local levels (levels of the
+// dependency graph restricted to the block), separated by workgroup
+// barriers, so dependencies inside the block cost an LDS read instead of a
+// device-scope round trip.  Dependencies on earlier blocks (later ones for
+// the backward sweep) are value-as-flag waits on the published results;
+// blocks are taken in dependency order through tickets, so every such wait
+// is on a resident or finished workgroup.  Per row, lane t of a G-lane group
+// accumulates entries t, t+G, ... in order and the group sum is the same
+// butterfly as k_sweep: results are bit-identical to k_sweep with the same G.
+// perm: rows ordered by (processing block, local level, row); groups of one
+// (block, level) are perm[lev_off[g] .. lev_off[g+1]); the groups of
+// processing block p are g in [blk_lev[p], blk_lev[p+1]).
+// ---------------------------------------------------------------------
+template <int G, typename T, bool UPPER>
+__global__ __launch_bounds__(512) void k_sweep_blk(int n, int blk, int nblk, const int* __restrict__ perm,
+                                                   const int* __restrict__ blk_lev, const int* __restrict__ lev_off,
+                                                   const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const int* __restrict__ dg, const T* __restrict__ val,
+                                                   const double* b, double* out, double* reset, const int* gate,
+                                                   int* ctr, int* err, uint64_t wd_limit) {
+    extern __shared__ double xs[];
+    if (gate && *gate == 0) return;
+    const int ticket = take_ticket(ctr);
+    const Watchdog wd(wd_limit);
+    const int bid = UPPER ? nblk - 1 - ticket : ticket;
+    const int bs = bid * blk;
+    const int be = bs + blk < n ? bs + blk : n;
+    const int ng = blockDim.x / G;
+    const int gid = threadIdx.x / G;
+    const int t = threadIdx.x & (G - 1);
+    bool timed_out = false;
+    for (int g = blk_lev[ticket]; g < blk_lev[ticket + 1]; ++g) {
+        const int lo = lev_off[g], hi = lev_off[g + 1];
+        for (int k = lo + gid; k < hi; k += ng) {
+            const int row = perm[k];
+            const int d = dg[row];
+            int e = (UPPER ? d + 1 : rp[row]) + t;
+            const int end = UPPER ? rp[row + 1] : d;
+            double acc = 0.0;
+            if (t == 0) {
+                acc = b[row];
+                if (reset) reset[row] = __longlong_as_double(kPending);
+            }
+            for (; e < end; e += G) {
+                const int c = ci[e];
+                double v;
+                if (c >= bs && c < be) {
+                    v = xs[c - bs];
+                } else {
+                    uint64_t bits;
+                    while ((bits = peek_bits(out + c)) == kPending) {
+                        if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
+                        spin_pause();
+                    }
+                    v = __longlong_as_double(bits);
+                }
+                acc = fma(-v, static_cast<double>(val[e]), acc);
+            }
+            const double s = group_sum<G>(acc);
+            if (t == 0) {
+                const double r = UPPER ? s / static_cast<double>(val[d]) : s;
+                xs[row - bs] = r;
+                publish(out + row, r);
+            }
+        }
+        __syncthreads();
+    }
+    if (timed_out) atomicOr(err, kErrTimeout);
     retire_ticket(ctr);
 }
 

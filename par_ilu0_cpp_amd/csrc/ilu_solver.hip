@@ -84,10 +84,10 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
     const int grid = grid_for(n, kBlock);
     if (upper)
         k_levels<true><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
-                                               e->err.get() + kErrLevels, wd);
+                                               e->err.get() + kErrLevels, wd, 0);
     else
         k_levels<false><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
-                                                e->err.get() + kErrLevels, wd);
+                                                e->err.get() + kErrLevels, wd, 0);
     PILU_HIP(hipGetLastError());
     const int nlev = read_int(maxlev, s) + 1;
     if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
@@ -106,6 +106,73 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
                                        perm.get(), static_cast<unsigned>(n), 0u, end_bit, s));
     PILU_HIP(hipStreamSynchronize(s));
     return nlev;
+}
+
+__global__ void k_block_keys(int n, int blk, int nblk, const int* __restrict__ level, bool upper, int lbits,
+                             unsigned* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned b = static_cast<unsigned>(i / blk);
+    const unsigned p = upper ? static_cast<unsigned>(nblk) - 1u - b : b;
+    keys[i] = (p << lbits) | static_cast<unsigned>(level[i]);
+}
+
+// Schedules of the block-local sweep: local levels inside blocks of `blk`
+// rows, rows sorted by (processing block, local level, row), and the group
+// offsets of every (block, level).
+void build_block_schedule(Ext* e, bool upper, int blk, DevBuf<int>& perm, DevBuf<int>& blev, DevBuf<int>& loff) {
+    const int n = e->n;
+    hipStream_t s = e->stream;
+    const int nblk = ceil_div(n, blk);
+    int* level = e->level.get();
+    int* maxlev = e->err.get() + kErrMaxLev;
+    k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
+    PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
+    const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
+    if (upper)
+        k_levels<true><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev,
+                                                              e->ctr.get(), e->err.get() + kErrLevels, wd, blk);
+    else
+        k_levels<false><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level,
+                                                               maxlev, e->ctr.get(), e->err.get() + kErrLevels, wd,
+                                                               blk);
+    PILU_HIP(hipGetLastError());
+    const int nlev = read_int(maxlev, s) + 1;
+    if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
+        throw std::runtime_error("block level analysis: dependency wait timed out (watchdog)");
+    int lbits = 1, pbits = 1;
+    while ((1 << lbits) < nlev) ++lbits;
+    while ((1 << pbits) < nblk) ++pbits;
+    if (lbits + pbits > 31) throw std::runtime_error("block schedule: too many blocks x levels");
+    DevBuf<unsigned> keys(n), keys_out(n);
+    DevBuf<int> rows(n);
+    k_block_keys<<<grid_for(n, 256), 256, 0, s>>>(n, blk, nblk, level, upper, lbits, keys.get());
+    k_iota<<<grid_for(n, 256), 256, 0, s>>>(n, rows.get());
+    perm.reserve(n);
+    size_t bytes = 0;
+    const unsigned end_bit = static_cast<unsigned>(lbits + pbits);
+    PILU_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.get(), keys_out.get(), rows.get(), perm.get(),
+                                       static_cast<unsigned>(n), 0u, end_bit, s));
+    e->sort_tmp.reserve(bytes);
+    PILU_HIP(rocprim::radix_sort_pairs(e->sort_tmp.get(), bytes, keys.get(), keys_out.get(), rows.get(), perm.get(),
+                                       static_cast<unsigned>(n), 0u, end_bit, s));
+    std::vector<unsigned> hk(static_cast<size_t>(n));
+    PILU_HIP(hipMemcpyAsync(hk.data(), keys_out.get(), sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    std::vector<int> off, bl(static_cast<size_t>(nblk) + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        if (i == 0 || hk[i] != hk[i - 1]) {
+            const unsigned p = hk[i] >> lbits;
+            if (i == 0 || p != (hk[i - 1] >> lbits)) bl[p] = static_cast<int>(off.size());
+            off.push_back(i);
+        }
+    }
+    off.push_back(n);
+    bl[nblk] = static_cast<int>(off.size()) - 1;
+    blev.reserve(bl.size());
+    loff.reserve(off.size());
+    PILU_HIP(hipMemcpy(blev.get(), bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(loff.get(), off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
 }
 
 // Device pattern is in e->rp / e->ci: validate, locate diagonals, schedule.
@@ -150,6 +217,18 @@ void analyse(Ext* e, const int* host_rp) {
     e->y.reserve(n);
     e->y_armed = false;
     e->factored = false;
+    e->mode_l = e->mode_u = 0;
+    e->blk_rows = e->nblk = 0;
+}
+
+// Blocks of blk rows for the block-local sweeps (both directions).
+void use_blocks(Ext* e, int blk) {
+    if (blk == e->blk_rows) return;
+    if (blk < 64 || blk > 16384) throw std::runtime_error("block rows must be in [64, 16384]");
+    build_block_schedule(e, false, blk, e->bperm_l, e->blev_l, e->loff_l);
+    build_block_schedule(e, true, blk, e->bperm_u, e->blev_u, e->loff_u);
+    e->blk_rows = blk;
+    e->nblk = ceil_div(e->n, blk);
 }
 
 template <int G, typename T, bool UPPER>
@@ -162,9 +241,40 @@ void sweep_one(Ext* e, const T* val, const double* b, double* out, double* reset
                                                         watchdog_ticks(kWatchdogSeconds));
 }
 
+template <int G, typename T, bool UPPER>
+void sweep_blk_one(Ext* e, const T* val, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+    const size_t lds = sizeof(double) * static_cast<size_t>(e->blk_rows);
+    auto kern = k_sweep_blk<G, T, UPPER>;
+    PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(lds)));
+    kern<<<e->nblk, 512, lds, e->stream>>>(e->n, e->blk_rows, e->nblk, UPPER ? e->bperm_u.get() : e->bperm_l.get(),
+                                          UPPER ? e->blev_u.get() : e->blev_l.get(),
+                                          UPPER ? e->loff_u.get() : e->loff_l.get(), e->rp.get(), e->ci.get(),
+                                          e->dg.get(), val, b, out, reset, gate, ctr, e->err.get() + kErrSweep,
+                                          watchdog_ticks(kWatchdogSeconds));
+}
+
+template <typename T, bool UPPER>
+void sweep_blk_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
+                        const int* gate, int* ctr) {
+    switch (lanes) {
+        case 1: sweep_blk_one<1, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 2: sweep_blk_one<2, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 4: sweep_blk_one<4, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 8: sweep_blk_one<8, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 16: sweep_blk_one<16, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        case 32: sweep_blk_one<32, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+        default: sweep_blk_one<64, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
+    }
+}
+
 template <typename T, bool UPPER>
 void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
                     const int* gate, int* ctr) {
+    if ((UPPER ? e->mode_u : e->mode_l) == 1 && e->blk_rows > 0) {
+        sweep_blk_dispatch<T, UPPER>(lanes, e, val, b, out, reset, gate, ctr);
+        return;
+    }
     switch (lanes) {
         case 1: sweep_one<1, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
         case 2: sweep_one<2, T, UPPER>(e, val, b, out, reset, gate, ctr); break;
@@ -367,58 +477,127 @@ void IluSolver::SetDiagonalCompensation(const float* d_omega) {
     ext_->omega_ptr = d_omega;
 }
 
-int IluSolver::AutotuneSweeps(int reps, double* table) {
+}  // namespace lljbash
+
+namespace pilu {
+namespace {
+// Average ms of the forward and backward sweeps under the current settings.
+void time_sweeps(Ext* e, int lanes, const double* b, double* x, int reps, hipEvent_t* ev, double& tl, double& tu) {
+    hipStream_t s = e->stream;
+    int* ctr = e->ctr.get() + 4;
+    e->lanes_l = e->lanes_u = lanes;
+    ilu_apply(e, b, x, nullptr);  // warm-up; also arms the buffers
+    tl = tu = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        PILU_HIP(hipEventRecord(ev[0], s));
+        if (e->precision == 32)
+            sweep_dispatch<float, false>(lanes, e, e->fac32.get(), b, e->y.get(), x, nullptr, ctr);
+        else
+            sweep_dispatch<double, false>(lanes, e, e->fac.get(), b, e->y.get(), x, nullptr, ctr);
+        PILU_HIP(hipEventRecord(ev[1], s));
+        if (e->precision == 32)
+            sweep_dispatch<float, true>(lanes, e, e->fac32.get(), e->y.get(), x, e->y.get(), nullptr, ctr + 2);
+        else
+            sweep_dispatch<double, true>(lanes, e, e->fac.get(), e->y.get(), x, e->y.get(), nullptr, ctr + 2);
+        PILU_HIP(hipEventRecord(ev[2], s));
+        PILU_HIP(hipEventSynchronize(ev[2]));
+        float a = 0.f, c = 0.f;
+        PILU_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        PILU_HIP(hipEventElapsedTime(&c, ev[1], ev[2]));
+        tl += a;
+        tu += c;
+    }
+    tl /= reps;
+    tu /= reps;
+}
+}  // namespace
+}  // namespace pilu
+
+namespace lljbash {
+
+int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     Ext* e = pilu::need_setup(ext_);
     if (!e->factored) throw std::runtime_error("IluSolver: Factorize before AutotuneSweeps");
+    reps = std::max(reps, 1);
     hipStream_t s = e->stream;
     static const int kLanes[7] = {1, 2, 4, 8, 16, 32, 64};
+    static const int kBlkLanes[4] = {4, 8, 16, 32};
+    static const int kBlocks[3] = {4096, 8192, 16384};
     pilu::DevBuf<double> b(e->n), x(e->n);
     pilu::k_fill_u64<<<pilu::grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(b.get()),
                                                                    0x3FF0000000000000ull);  // 1.0
     hipEvent_t ev[3];
     for (auto& v : ev) PILU_HIP(hipEventCreate(&v));
-    const int save_l = e->lanes_l, save_u = e->lanes_u;
-    double best[2] = {1e300, 1e300};
-    int pick[2] = {8, 8};
-    for (int k = 0; k < 7; ++k) {
-        e->lanes_l = e->lanes_u = kLanes[k];
-        pilu::ilu_apply(e, b.get(), x.get(), nullptr);  // warm
-        double tl = 0.0, tu = 0.0;
-        for (int r = 0; r < std::max(reps, 1); ++r) {
-            int* ctr = e->ctr.get() + 4;
-            PILU_HIP(hipEventRecord(ev[0], s));
-            if (e->precision == 32)
-                pilu::sweep_dispatch<float, false>(kLanes[k], e, e->fac32.get(), b.get(), e->y.get(), x.get(), nullptr, ctr);
-            else
-                pilu::sweep_dispatch<double, false>(kLanes[k], e, e->fac.get(), b.get(), e->y.get(), x.get(), nullptr, ctr);
-            PILU_HIP(hipEventRecord(ev[1], s));
-            if (e->precision == 32)
-                pilu::sweep_dispatch<float, true>(kLanes[k], e, e->fac32.get(), e->y.get(), x.get(), e->y.get(), nullptr,
-                                                  ctr + 2);
-            else
-                pilu::sweep_dispatch<double, true>(kLanes[k], e, e->fac.get(), e->y.get(), x.get(), e->y.get(), nullptr,
-                                                   ctr + 2);
-            PILU_HIP(hipEventRecord(ev[2], s));
-            PILU_HIP(hipEventSynchronize(ev[2]));
-            float a = 0.f, c = 0.f;
-            PILU_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-            PILU_HIP(hipEventElapsedTime(&c, ev[1], ev[2]));
-            tl += a;
-            tu += c;
+    int rows = 0;
+    auto record = [&](int mode, int blk, int lanes, double tl, double tu) {
+        if (table && rows < cap) {
+            double* r = table + 5 * rows;
+            r[0] = mode; r[1] = blk; r[2] = lanes; r[3] = tl; r[4] = tu;
         }
-        tl /= std::max(reps, 1);
-        tu /= std::max(reps, 1);
-        if (table) { table[2 * k] = tl; table[2 * k + 1] = tu; }
-        if (tl < best[0]) { best[0] = tl; pick[0] = kLanes[k]; }
-        if (tu < best[1]) { best[1] = tu; pick[1] = kLanes[k]; }
+        ++rows;
+    };
+    // level-scheduled rows
+    e->mode_l = e->mode_u = 0;
+    double lev_best[2] = {1e300, 1e300};
+    int lev_lanes[2] = {8, 8};
+    for (int g : kLanes) {
+        double tl, tu;
+        pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+        record(0, 0, g, tl, tu);
+        if (tl < lev_best[0]) { lev_best[0] = tl; lev_lanes[0] = g; }
+        if (tu < lev_best[1]) { lev_best[1] = tu; lev_lanes[1] = g; }
+    }
+    // block-local, per block size
+    double best_total = lev_best[0] + lev_best[1];
+    int pick_mode[2] = {0, 0}, pick_lanes[2] = {lev_lanes[0], lev_lanes[1]}, pick_blk = 0;
+    for (int blk : kBlocks) {
+        if (blk > e->n) break;
+        pilu::use_blocks(e, blk);
+        e->mode_l = e->mode_u = 1;
+        double bb[2] = {1e300, 1e300};
+        int bl[2] = {8, 8};
+        for (int g : kBlkLanes) {
+            double tl, tu;
+            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+            record(1, blk, g, tl, tu);
+            if (tl < bb[0]) { bb[0] = tl; bl[0] = g; }
+            if (tu < bb[1]) { bb[1] = tu; bl[1] = g; }
+        }
+        const double tot = std::min(bb[0], lev_best[0]) + std::min(bb[1], lev_best[1]);
+        if (tot < best_total) {
+            best_total = tot;
+            pick_blk = blk;
+            for (int d = 0; d < 2; ++d) {
+                pick_mode[d] = bb[d] < lev_best[d] ? 1 : 0;
+                pick_lanes[d] = bb[d] < lev_best[d] ? bl[d] : lev_lanes[d];
+            }
+        }
     }
     for (auto& v : ev) (void)hipEventDestroy(v);
     pilu::check_sweep_errors(e);
-    (void)save_l;
-    (void)save_u;
-    e->lanes_l = pick[0];
-    e->lanes_u = pick[1];
-    return pick[0] * 100 + pick[1];
+    if (pick_blk > 0) pilu::use_blocks(e, pick_blk);
+    e->mode_l = pick_mode[0];
+    e->mode_u = pick_mode[1];
+    e->lanes_l = pick_lanes[0];
+    e->lanes_u = pick_lanes[1];
+    return rows;
+}
+
+void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
+    Ext* e = pilu::need_setup(ext_);
+    if ((mode_l != 0 && mode_l != 1) || (mode_u != 0 && mode_u != 1)) throw std::runtime_error("sweep mode is 0 or 1");
+    if (mode_l == 1 || mode_u == 1) pilu::use_blocks(e, blk_rows);
+    e->mode_l = mode_l;
+    e->mode_u = mode_u;
+}
+
+void IluSolver::GetSweepConfig(int* cfg) const {
+    const Ext* e = ext_;
+    cfg[0] = e ? e->mode_l : 0;
+    cfg[1] = e ? pilu::effective_lanes(e, false) : 0;
+    cfg[2] = e ? e->mode_u : 0;
+    cfg[3] = e ? pilu::effective_lanes(e, true) : 0;
+    cfg[4] = e ? e->blk_rows : 0;
 }
 
 void IluSolver::SetFactorPrecision(int bits) {
