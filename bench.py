@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--forms", default="fp64,fp32")
     p.add_argument("--cpu-steps", type=int, default=12,
                    help="Arnoldi steps in the bounded CPU baseline sample (0 = skip)")
+    p.add_argument("--no-autotune", action="store_true", help="keep the default sweep shape")
+    p.add_argument("--traffic", default="", help="JSON file with measured HBM bytes per apply (rocprof)")
     return p.parse_args()
 
 
@@ -106,6 +108,15 @@ def main():
             raise SystemExit(f"symbolic failed: {nnz_f}")
         nl, nu = ilu.Levels()
         log(f"[bench] {form}: symbolic+analysis {t_setup*1e3:.1f} ms nnz(F)={nnz_f} levels L={nl} U={nu}")
+        sweep_cfg = ilu.SweepConfig()
+        if not args.no_autotune:
+            # one factorization, then time every sweep shape (part of setup, untimed)
+            if ilu.FactorizeDevice(d_va.data_ptr(), True) != 0:
+                raise SystemExit("factorization failed")
+            ta = time.perf_counter()
+            sweep_cfg, _ = ilu.Autotune(reps=2)
+            t_setup += time.perf_counter() - ta
+            log(f"[bench] {form}: autotuned sweeps {sweep_cfg} in {time.perf_counter() - ta:.2f}s")
         g = P.PreconditionedGmres()
         g.SetIluPreconditioner(ilu)
         prm = g.param()
@@ -142,7 +153,7 @@ def main():
         bt = tri_bytes(a.n, nnz_f, bits // 8)
         results[form] = dict(
             s_per_solve=per, its=its_all, true_rel_res=max(res_all), ok=all(oks), setup_s=t_setup,
-            nnz_f=nnz_f, levels=[nl, nu], apply_s=t_apply, mv_s=t_mv, tri_bytes=bt,
+            nnz_f=nnz_f, levels=[nl, nu], apply_s=t_apply, mv_s=t_mv, tri_bytes=bt, sweep=sweep_cfg,
             tri_gbs=bt / t_apply / 1e9 if t_apply > 0 else None,
         )
         log(f"[bench] {form}: {per*1e3:.1f} ms/solve its={its_all} rel_res={max(res_all):.2e} "
@@ -176,7 +187,7 @@ def main():
             "form": main_form,
             "n": a.n, "nnz_a": a.nnz, "nnz_f": r["nnz_f"], "levels_l_u": r["levels"],
             "iterations": r["its"], "true_rel_res": r["true_rel_res"],
-            "setup_s": r["setup_s"], "parallelism": "single GPU",
+            "setup_s": r["setup_s"], "parallelism": "single GPU", "sweep_shape": r["sweep"],
         },
         "forms": {f: {"s_per_solve": v["s_per_solve"], "its": v["its"], "true_rel_res": v["true_rel_res"],
                       "apply_us": v["apply_s"] * 1e6, "trisolve_gbs": v["tri_gbs"]} for f, v in results.items()},
@@ -189,13 +200,23 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (r["tri_gbs"] / HBM_PEAK_GBS) if r["tri_gbs"] else None,
-            "traffic": None,
+            "traffic": traffic_bytes(args.traffic, main_form),
             "bytes_per_apply": r["tri_bytes"],
             "apply_us": r["apply_s"] * 1e6,
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def traffic_bytes(path, form):
+    """HBM bytes per apply measured by rocprofv3 FETCH_SIZE + WRITE_SIZE
+    (calibrated against a copy kernel), recorded in `path` by the profiling
+    run; None when not measured."""
+    if not path or not Path(path).exists():
+        return None
+    d = json.loads(Path(path).read_text())
+    return d.get(form)
 
 
 def cpu_baseline(a, b, tol, gpu_its, steps):

@@ -67,14 +67,15 @@ public:
     void SetDiagonalCompensation(const float* d_omega);
     // Times the sweeps in every launch shape -- level-scheduled rows with
     // 1..64 lanes per row, and block-local sweeps (blocks of 4096, 8192 or
-    // 16384 rows staged in LDS) with 4..32 lanes -- over `reps` applies
+    // 16384 rows staged in LDS; CSR or packed factors) -- over `reps` applies
     // each, and keeps the fastest shape for each direction.  Results depend
     // on the lanes per row only through the rounding of row sums.
     // table (optional, cap rows of 5 doubles): mode, block rows, lanes,
     // forward ms, backward ms.  Returns the number of shapes timed.
     int AutotuneSweeps(int reps, double* table, int cap);
     // Sweep algorithm per direction: 0 level-scheduled rows, 1 block-local
-    // with blocks of blk_rows rows (64..16384).
+    // with blocks of blk_rows rows (64..16384), 2 block-local over a packed,
+    // schedule-ordered copy of the factors (refreshed at every Factorize).
     void SetSweepMode(int mode_l, int mode_u, int blk_rows);
     // cfg[5]: mode_l, lanes_l, mode_u, lanes_u, blk_rows.
     void GetSweepConfig(int* cfg) const;

@@ -4,6 +4,8 @@
 #include "common.hpp"
 #include "ilu_solver.hpp"
 
+#include <vector>
+
 namespace pilu {
 
 // Launch-shape choice for one sweep direction.
@@ -25,6 +27,24 @@ void check_sweep_errors(lljbash::IluSolver::Ext* ext);
 // heuristic; the autotuner may override it).
 int lanes_for_avg(double avg_len);
 
+}  // namespace pilu
+
+namespace pilu {
+struct PackPass;
+// Schedule-ordered copy of one direction's factors for the packed sweep.
+struct SweepPack {
+    DevBuf<int> rowinfo;            // (row, entry count) per schedule position
+    DevBuf<int> pcol, psrc;         // slab columns and source CSR positions
+    DevBuf<double> pval64, pdiag64;
+    DevBuf<float> pval32, pdiag32;
+    std::vector<int> h_blev, h_loff, h_base, h_width;  // host copies for pass lists
+    long ne = 0;
+    int max_w = 0;
+    int blk = 0;                    // block rows the pack was built for
+    DevBuf<int> pass_ptr;
+    DevBuf<char> passes;            // PackPass records
+    int pass_g = 0;                 // lanes per row the pass list was built for
+};
 }  // namespace pilu
 
 struct lljbash::IluSolver::Ext {
@@ -53,8 +73,10 @@ struct lljbash::IluSolver::Ext {
     int precision = 64;
     int lanes_l = 0, lanes_u = 0;  // 0 = heuristic
     // Sweep algorithm per direction: 0 = level-scheduled rows (k_sweep),
-    // 1 = block-local (k_sweep_blk) with blocks of blk_rows rows.
+    // 1 = block-local (k_sweep_blk) with blocks of blk_rows rows,
+    // 2 = packed block-local (k_sweep_pk).
     int mode_l = 0, mode_u = 0;
+    pilu::SweepPack pack_l, pack_u;
     int blk_rows = 0, nblk = 0;
     pilu::DevBuf<int> bperm_l, bperm_u, blev_l, blev_u, loff_l, loff_u;
     bool factored = false;

@@ -274,6 +274,144 @@ __global__ __launch_bounds__(512) void k_sweep_blk(int n, int blk, int nblk, con
 }
 
 // ---------------------------------------------------------------------
+// Packed block-local sweep.  Same schedule as k_sweep_blk (blocks of blk
+// rows, local levels, LDS-resident block results), but the factors are
+// also stored in schedule order: for every (block, local level) group the
+// entries of its rows form a column-major ELL slab (entry q of the group's
+// j-th row at base + q * count + j), so a level needs one round of loads
+// that depends on nothing but the pass record.  Each workgroup walks its
+// block as a list of passes (up to 512/G rows of one level group) and loads
+// pass p+1 while computing pass p; workgroup barriers, LDS-only, sit at
+// level boundaries.  Row sums follow k_sweep's lane order: bit-identical
+// results for the same G.
+// ---------------------------------------------------------------------
+struct PackPass {
+    int k_lo, k_hi;  // schedule positions of the pass
+    int gstart;      // first position of the pass's level group
+    int base;        // slab offset of the group
+    int count;       // rows in the group (slab column stride)
+    int width;       // entries of the group's longest row
+    int bar;         // 1: a level boundary follows this pass
+    int pad;
+};
+
+template <int S, typename T>
+struct PackRow {
+    int row = 0, cnt = 0, j = 0;
+    bool act = false;
+    double bv = 0.0;
+    T dv = T(1);
+    int c[S];
+    T v[S];
+};
+
+template <int G, int S, typename T, bool UPPER>
+__device__ __forceinline__ void pk_load(const PackPass& pr, int gid, int t, const int2* __restrict__ rowinfo,
+                                        const int* __restrict__ pcol, const T* __restrict__ pval,
+                                        const T* __restrict__ pdiag, const double* b, PackRow<S, T>& r) {
+    const int k = pr.k_lo + gid;
+    r.act = k < pr.k_hi;
+    if (!r.act) return;
+    r.j = k - pr.gstart;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int q = t + s * G;
+        if (q < pr.width) {
+            const int pos = pr.base + q * pr.count + r.j;
+            r.c[s] = pcol[pos];
+            r.v[s] = pval[pos];
+        }
+    }
+    const int2 ri = rowinfo[k];
+    r.row = ri.x;
+    r.cnt = ri.y;
+    if (UPPER) r.dv = pdiag[k];
+    if (t == 0) r.bv = b[r.row];
+}
+
+template <int G, int S, typename T, bool UPPER>
+__global__ __launch_bounds__(512) void k_sweep_pk(int n, int blk, int nblk, const int* __restrict__ pass_ptr,
+                                                  const PackPass* __restrict__ passes,
+                                                  const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
+                                                  const T* __restrict__ pval, const T* __restrict__ pdiag,
+                                                  const double* b, double* out, double* reset, const int* gate,
+                                                  int* ctr, int* err, uint64_t wd_limit) {
+    extern __shared__ double xs[];
+    if (gate && *gate == 0) return;
+    const int ticket = take_ticket(ctr);
+    const Watchdog wd(wd_limit);
+    const int bid = UPPER ? nblk - 1 - ticket : ticket;
+    const int bs = bid * blk;
+    const int be = bs + blk < n ? bs + blk : n;
+    const int gid = threadIdx.x / G;
+    const int t = threadIdx.x & (G - 1);
+    const int p0 = pass_ptr[ticket], p1 = pass_ptr[ticket + 1];
+    bool timed_out = false;
+    PackRow<S, T> cur, nxt;
+    PackPass pc = passes[p0];
+    pk_load<G, S, T, UPPER>(pc, gid, t, rowinfo, pcol, pval, pdiag, b, cur);
+    for (int p = p0; p < p1; ++p) {
+        PackPass pn = pc;
+        if (p + 1 < p1) {
+            pn = passes[p + 1];
+            pk_load<G, S, T, UPPER>(pn, gid, t, rowinfo, pcol, pval, pdiag, b, nxt);
+        }
+        if (cur.act) {
+            double acc = t == 0 ? cur.bv : 0.0;
+            if (t == 0 && reset) reset[cur.row] = __longlong_as_double(kPending);
+            auto fetch_x = [&](int c) -> double {
+                if (c >= bs && c < be) return xs[c - bs];
+                uint64_t bits;
+                while ((bits = peek_bits(out + c)) == kPending) {
+                    if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
+                    spin_pause();
+                }
+                return __longlong_as_double(bits);
+            };
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (t + s * G < cur.cnt) acc = fma(-fetch_x(cur.c[s]), static_cast<double>(cur.v[s]), acc);
+            }
+            for (int q = t + S * G; q < cur.cnt; q += G) {
+                const int pos = pc.base + q * pc.count + cur.j;
+                acc = fma(-fetch_x(pcol[pos]), static_cast<double>(pval[pos]), acc);
+            }
+            const double sum = group_sum<G>(acc);
+            if (t == 0) {
+                const double r = UPPER ? sum / static_cast<double>(cur.dv) : sum;
+                xs[cur.row - bs] = r;
+                publish(out + cur.row, r);
+            }
+        }
+        if (pc.bar) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        }
+        pc = pn;
+        cur = nxt;
+    }
+    if (timed_out) atomicOr(err, kErrTimeout);
+    retire_ticket(ctr);
+}
+
+// Values of the packed sweep from the factors: pval[e] = fac[src[e]] (src
+// < 0: padding), pdiag[k] = fac[dg[row of k]].
+template <typename TI, typename TO>
+__global__ void k_pack_gather(long ne, const int* __restrict__ src, const TI* __restrict__ fac, TO* __restrict__ pval,
+                              int n, const int2* __restrict__ rowinfo, const int* __restrict__ dg,
+                              TO* __restrict__ pdiag) {
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; e < ne; e += stride) {
+        const int s = src[e];
+        pval[e] = s >= 0 ? static_cast<TO>(fac[s]) : TO(0);
+    }
+    if (pdiag)
+        for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < n; k += stride)
+            pdiag[k] = static_cast<TO>(fac[dg[rowinfo[k].x]]);
+}
+
+// ---------------------------------------------------------------------
 // Numeric ILU, row IKJ form of left_looking_col (src/ilu_solver.cpp:553-589):
 //   for pivots k of row i in ascending column order:
 //     a_ik /= u_kk;  a_ij -= u_kj * a_ik  for j in U(k), j > k, j in row i

@@ -221,6 +221,130 @@ void analyse(Ext* e, const int* host_rp) {
     e->blk_rows = e->nblk = 0;
 }
 
+// Packed schedule-ordered factors of one direction for the current blocks.
+void build_pack(Ext* e, bool upper, SweepPack& pk) {
+    const int n = e->n;
+    const DevBuf<int>& perm = upper ? e->bperm_u : e->bperm_l;
+    const DevBuf<int>& blev = upper ? e->blev_u : e->blev_l;
+    const DevBuf<int>& loff = upper ? e->loff_u : e->loff_l;
+    std::vector<int> rp(static_cast<size_t>(n) + 1), dg(n), pm(n), ci(static_cast<size_t>(e->nnz));
+    PILU_HIP(hipMemcpy(rp.data(), e->rp.get(), sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+    PILU_HIP(hipMemcpy(dg.data(), e->dg.get(), sizeof(int) * dg.size(), hipMemcpyDeviceToHost));
+    PILU_HIP(hipMemcpy(pm.data(), perm.get(), sizeof(int) * pm.size(), hipMemcpyDeviceToHost));
+    PILU_HIP(hipMemcpy(ci.data(), e->ci.get(), sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+    pk.h_blev.assign(static_cast<size_t>(e->nblk) + 1, 0);
+    PILU_HIP(hipMemcpy(pk.h_blev.data(), blev.get(), sizeof(int) * pk.h_blev.size(), hipMemcpyDeviceToHost));
+    const int ngroups = pk.h_blev[e->nblk];
+    pk.h_loff.assign(static_cast<size_t>(ngroups) + 1, 0);
+    PILU_HIP(hipMemcpy(pk.h_loff.data(), loff.get(), sizeof(int) * pk.h_loff.size(), hipMemcpyDeviceToHost));
+    pk.h_base.assign(ngroups, 0);
+    pk.h_width.assign(ngroups, 0);
+    std::vector<int> rowinfo(2 * static_cast<size_t>(n));
+    long total = 0;
+    int maxw = 0;
+    for (int g = 0; g < ngroups; ++g) {
+        int w = 0;
+        for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
+            const int row = pm[k];
+            const int cnt = upper ? rp[row + 1] - dg[row] - 1 : dg[row] - rp[row];
+            rowinfo[2 * static_cast<size_t>(k)] = row;
+            rowinfo[2 * static_cast<size_t>(k) + 1] = cnt;
+            w = std::max(w, cnt);
+        }
+        pk.h_base[g] = static_cast<int>(total);
+        pk.h_width[g] = w;
+        maxw = std::max(maxw, w);
+        total += static_cast<long>(w) * (pk.h_loff[g + 1] - pk.h_loff[g]);
+        if (total >= (1L << 31)) throw std::runtime_error("packed sweep: too many entries");
+    }
+    std::vector<int> pcol(std::max<long>(total, 1), 0), psrc(std::max<long>(total, 1), -1);
+    for (int g = 0; g < ngroups; ++g) {
+        const int count = pk.h_loff[g + 1] - pk.h_loff[g];
+        for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
+            const int row = pm[k];
+            const int cnt = rowinfo[2 * static_cast<size_t>(k) + 1];
+            const int e0 = upper ? dg[row] + 1 : rp[row];
+            for (int q = 0; q < cnt; ++q) {
+                const long pos = pk.h_base[g] + static_cast<long>(q) * count + (k - pk.h_loff[g]);
+                pcol[pos] = ci[e0 + q];
+                psrc[pos] = e0 + q;
+            }
+        }
+    }
+    pk.ne = total;
+    pk.max_w = maxw;
+    pk.rowinfo.resize(rowinfo.size());
+    pk.pcol.resize(pcol.size());
+    pk.psrc.resize(psrc.size());
+    PILU_HIP(hipMemcpy(pk.rowinfo.get(), rowinfo.data(), sizeof(int) * rowinfo.size(), hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(pk.pcol.get(), pcol.data(), sizeof(int) * pcol.size(), hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(pk.psrc.get(), psrc.data(), sizeof(int) * psrc.size(), hipMemcpyHostToDevice));
+    pk.blk = e->blk_rows;
+    pk.pass_g = 0;
+}
+
+// Pass list of a pack for G lanes per row (512 / G rows per pass).
+void build_passes(Ext* e, SweepPack& pk, int G) {
+    if (pk.pass_g == G) return;
+    const int per = 512 / G;
+    std::vector<PackPass> passes;
+    std::vector<int> ptr(static_cast<size_t>(e->nblk) + 1, 0);
+    for (int p = 0; p < e->nblk; ++p) {
+        ptr[p] = static_cast<int>(passes.size());
+        for (int g = pk.h_blev[p]; g < pk.h_blev[p + 1]; ++g) {
+            const int lo = pk.h_loff[g], hi = pk.h_loff[g + 1];
+            for (int c = lo; c < hi; c += per) {
+                PackPass r{};
+                r.k_lo = c;
+                r.k_hi = std::min(c + per, hi);
+                r.gstart = lo;
+                r.base = pk.h_base[g];
+                r.count = hi - lo;
+                r.width = pk.h_width[g];
+                r.bar = c + per >= hi ? 1 : 0;
+                passes.push_back(r);
+            }
+        }
+    }
+    ptr[e->nblk] = static_cast<int>(passes.size());
+    pk.pass_ptr.resize(ptr.size());
+    pk.passes.resize(sizeof(PackPass) * std::max<size_t>(passes.size(), 1));
+    PILU_HIP(hipMemcpy(pk.pass_ptr.get(), ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(pk.passes.get(), passes.data(), sizeof(PackPass) * passes.size(), hipMemcpyHostToDevice));
+    pk.pass_g = G;
+}
+
+// Refresh the packed values from the current factors.
+void gather_pack(Ext* e, SweepPack& pk, bool upper) {
+    if (pk.ne == 0 || pk.blk != e->blk_rows) return;
+    hipStream_t s = e->stream;
+    const int grid = 2048;
+    if (e->precision == 32) {
+        pk.pval32.reserve(pk.ne);
+        if (upper) pk.pdiag32.reserve(e->n);
+        k_pack_gather<float, float><<<grid, 256, 0, s>>>(pk.ne, pk.psrc.get(), e->fac32.get(), pk.pval32.get(), e->n,
+                                                         reinterpret_cast<const int2*>(pk.rowinfo.get()), e->dg.get(),
+                                                         upper ? pk.pdiag32.get() : nullptr);
+    } else {
+        pk.pval64.reserve(pk.ne);
+        if (upper) pk.pdiag64.reserve(e->n);
+        k_pack_gather<double, double><<<grid, 256, 0, s>>>(pk.ne, pk.psrc.get(), e->fac.get(), pk.pval64.get(), e->n,
+                                                           reinterpret_cast<const int2*>(pk.rowinfo.get()),
+                                                           e->dg.get(), upper ? pk.pdiag64.get() : nullptr);
+    }
+    PILU_HIP(hipGetLastError());
+}
+
+void ensure_packs(Ext* e) {
+    if (e->blk_rows <= 0) throw std::runtime_error("packed sweep needs block rows");
+    if (e->pack_l.blk != e->blk_rows) build_pack(e, false, e->pack_l);
+    if (e->pack_u.blk != e->blk_rows) build_pack(e, true, e->pack_u);
+    if (e->factored) {
+        gather_pack(e, e->pack_l, false);
+        gather_pack(e, e->pack_u, true);
+    }
+}
+
 // Blocks of blk rows for the block-local sweeps (both directions).
 void use_blocks(Ext* e, int blk) {
     if (blk == e->blk_rows) return;
@@ -228,6 +352,7 @@ void use_blocks(Ext* e, int blk) {
     build_block_schedule(e, false, blk, e->bperm_l, e->blev_l, e->loff_l);
     build_block_schedule(e, true, blk, e->bperm_u, e->blev_u, e->loff_u);
     e->blk_rows = blk;
+    e->pack_l.blk = e->pack_u.blk = 0;  // packs are rebuilt on first use
     e->nblk = ceil_div(e->n, blk);
 }
 
@@ -268,9 +393,58 @@ void sweep_blk_dispatch(int lanes, Ext* e, const T* val, const double* b, double
     }
 }
 
+template <int G, int S, typename T, bool UPPER>
+void sweep_pk_one(Ext* e, const T* pval, const T* pdiag, const double* b, double* out, double* reset, const int* gate,
+                  int* ctr) {
+    SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
+    build_passes(e, pk, G);
+    const size_t lds = sizeof(double) * static_cast<size_t>(e->blk_rows);
+    auto kern = k_sweep_pk<G, S, T, UPPER>;
+    PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(lds)));
+    kern<<<e->nblk, 512, lds, e->stream>>>(e->n, e->blk_rows, e->nblk, pk.pass_ptr.get(),
+                                          reinterpret_cast<const PackPass*>(pk.passes.get()),
+                                          reinterpret_cast<const int2*>(pk.rowinfo.get()), pk.pcol.get(), pval, pdiag,
+                                          b, out, reset, gate, ctr, e->err.get() + kErrSweep,
+                                          watchdog_ticks(kWatchdogSeconds));
+}
+
+template <int G, typename T, bool UPPER>
+void sweep_pk_s(Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+    SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
+    const T* pval;
+    const T* pdiag;
+    if constexpr (sizeof(T) == 4) {
+        pval = pk.pval32.get();
+        pdiag = UPPER ? pk.pdiag32.get() : nullptr;
+    } else {
+        pval = pk.pval64.get();
+        pdiag = UPPER ? pk.pdiag64.get() : nullptr;
+    }
+    const int need = (pk.max_w + G - 1) / G;
+    if (need <= 1) sweep_pk_one<G, 1, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+    else if (need <= 2) sweep_pk_one<G, 2, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+    else sweep_pk_one<G, 4, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+}
+
+template <typename T, bool UPPER>
+void sweep_pk_dispatch(int lanes, Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+    switch (lanes) {
+        case 1: sweep_pk_s<1, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 2: sweep_pk_s<2, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 4: sweep_pk_s<4, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 8: sweep_pk_s<8, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        default: sweep_pk_s<16, T, UPPER>(e, b, out, reset, gate, ctr); break;
+    }
+}
+
 template <typename T, bool UPPER>
 void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
                     const int* gate, int* ctr) {
+    if ((UPPER ? e->mode_u : e->mode_l) == 2 && e->blk_rows > 0) {
+        sweep_pk_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
+        return;
+    }
     if ((UPPER ? e->mode_u : e->mode_l) == 1 && e->blk_rows > 0) {
         sweep_blk_dispatch<T, UPPER>(lanes, e, val, b, out, reset, gate, ctr);
         return;
@@ -323,6 +497,8 @@ bool run_factor(Ext* e) {
     if (err & kErrTimeout) throw std::runtime_error("device factorization: dependency wait timed out (watchdog)");
     if (err & 2) throw std::runtime_error("device factorization: row exceeded the LDS staging capacity");
     e->factored = true;
+    gather_pack(e, e->pack_l, false);
+    gather_pack(e, e->pack_u, true);
     return (err & 1) == 0;
 }
 
@@ -522,6 +698,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     hipStream_t s = e->stream;
     static const int kLanes[7] = {1, 2, 4, 8, 16, 32, 64};
     static const int kBlkLanes[4] = {4, 8, 16, 32};
+    static const int kPackLanes[4] = {2, 4, 8, 16};
     static const int kBlocks[3] = {4096, 8192, 16384};
     pilu::DevBuf<double> b(e->n), x(e->n);
     pilu::k_fill_u64<<<pilu::grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(b.get()),
@@ -563,12 +740,22 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
             if (tl < bb[0]) { bb[0] = tl; bl[0] = g; }
             if (tu < bb[1]) { bb[1] = tu; bl[1] = g; }
         }
+        int bm[2] = {1, 1};
+        pilu::ensure_packs(e);
+        e->mode_l = e->mode_u = 2;
+        for (int g : kPackLanes) {
+            double tl, tu;
+            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+            record(2, blk, g, tl, tu);
+            if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 2; }
+            if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 2; }
+        }
         const double tot = std::min(bb[0], lev_best[0]) + std::min(bb[1], lev_best[1]);
         if (tot < best_total) {
             best_total = tot;
             pick_blk = blk;
             for (int d = 0; d < 2; ++d) {
-                pick_mode[d] = bb[d] < lev_best[d] ? 1 : 0;
+                pick_mode[d] = bb[d] < lev_best[d] ? bm[d] : 0;
                 pick_lanes[d] = bb[d] < lev_best[d] ? bl[d] : lev_lanes[d];
             }
         }
@@ -576,6 +763,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     for (auto& v : ev) (void)hipEventDestroy(v);
     pilu::check_sweep_errors(e);
     if (pick_blk > 0) pilu::use_blocks(e, pick_blk);
+    if (pick_mode[0] == 2 || pick_mode[1] == 2) pilu::ensure_packs(e);
     e->mode_l = pick_mode[0];
     e->mode_u = pick_mode[1];
     e->lanes_l = pick_lanes[0];
@@ -585,8 +773,9 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
 
 void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
     Ext* e = pilu::need_setup(ext_);
-    if ((mode_l != 0 && mode_l != 1) || (mode_u != 0 && mode_u != 1)) throw std::runtime_error("sweep mode is 0 or 1");
-    if (mode_l == 1 || mode_u == 1) pilu::use_blocks(e, blk_rows);
+    if (mode_l < 0 || mode_l > 2 || mode_u < 0 || mode_u > 2) throw std::runtime_error("sweep mode is 0, 1 or 2");
+    if (mode_l > 0 || mode_u > 0) pilu::use_blocks(e, blk_rows);
+    if (mode_l == 2 || mode_u == 2) pilu::ensure_packs(e);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
 }

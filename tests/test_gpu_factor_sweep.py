@@ -168,27 +168,43 @@ def test_autotune_keeps_results(oracle):
     assert np.max(np.abs(x1 - x0)) <= 1e-12 * np.max(np.abs(x0))
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("blk", [64, 1000, 4096, 100000])
-@pytest.mark.parametrize("lanes", [1, 4, 8, 32])
-def test_block_sweep_bit_identical_to_level_sweep(oracle, blk, lanes):
-    """The block-local sweep sums each row in the same order as the
-    level-scheduled sweep with the same lanes per row: identical bits."""
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
+def test_block_sweep_bit_identical_to_level_sweep(oracle, mode, blk, lanes):
+    """The block-local sweeps (CSR or packed factors) sum each row in the
+    same order as the level-scheduled sweep with the same lanes per row:
+    identical bits, in either direction, applied repeatedly."""
     a = gen.convdiff3d(15)
     o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
     ilu = factored(o[0], o[1], o[3])
     b = np.random.default_rng(lanes).uniform(-1, 1, a.n)
     ilu.SetSweepLanes(lanes, lanes)
     x_level = ilu.Substitute(b)
-    ilu.SetSweepMode(1, 1, min(blk, 16384))
+    blk = min(blk, 16384)
+    ilu.SetSweepMode(mode, mode, blk)
     for _ in range(2):
         assert np.array_equal(ilu.Substitute(b), x_level)
-    ilu.SetSweepMode(1, 0, min(blk, 16384))
+    ilu.SetSweepMode(mode, 0, blk)
     assert np.array_equal(ilu.Substitute(b), x_level)
-    ilu.SetSweepMode(0, 1, min(blk, 16384))
+    ilu.SetSweepMode(0, mode, blk)
     assert np.array_equal(ilu.Substitute(b), x_level)
     if lanes == 1:
         ofac, dg, _ = oracle.ilu_factorize(o[0], o[1], o[3])
         assert np.array_equal(x_level, oracle.substitute(o[0], o[1], dg, ofac, b))
+
+
+def test_packed_sweep_follows_refactorization(oracle):
+    a = gen.convdiff3d(12)
+    ilu = factored(a.row_ptr, a.col_idx, a.value)
+    ilu.SetSweepMode(2, 2, 512)
+    va2 = a.value * np.linspace(1.0, 1.5, a.nnz)
+    P.fill_csr(ilu.GetMatrix(), a.row_ptr, a.col_idx, va2)
+    assert ilu.Factorize() == 0
+    b = np.random.default_rng(4).uniform(-1, 1, a.n)
+    ofac, dg, _ = oracle.ilu_factorize(a.row_ptr, a.col_idx, va2)
+    xo = oracle.substitute(a.row_ptr, a.col_idx, dg, ofac, b)
+    assert np.max(np.abs(ilu.Substitute(b) - xo)) <= 1e-12 * np.max(np.abs(xo))
 
 
 def test_block_sweep_random_pattern(oracle):
@@ -199,6 +215,7 @@ def test_block_sweep_random_pattern(oracle):
     b = np.random.default_rng(2).uniform(-1, 1, 5000)
     xo = oracle.substitute(rp, ci, dg, ofac, b)
     ilu.SetSweepLanes(4, 4)
-    ilu.SetSweepMode(1, 1, 512)
-    x = ilu.Substitute(b)
-    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+    for mode in (1, 2):
+        ilu.SetSweepMode(mode, mode, 512)
+        x = ilu.Substitute(b)
+        assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
