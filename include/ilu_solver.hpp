@@ -75,7 +75,8 @@ public:
     int AutotuneSweeps(int reps, double* table, int cap);
     // Sweep algorithm per direction: 0 level-scheduled rows, 1 block-local
     // with blocks of blk_rows rows (64..16384), 2 block-local over a packed,
-    // schedule-ordered copy of the factors (refreshed at every Factorize).
+    // schedule-ordered copy of the factors (refreshed at every Factorize),
+    // 3 the packed sweep with its loads issued several passes ahead.
     void SetSweepMode(int mode_l, int mode_u, int blk_rows);
     // cfg[5]: mode_l, lanes_l, mode_u, lanes_u, blk_rows.
     void GetSweepConfig(int* cfg) const;

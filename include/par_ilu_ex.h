@@ -34,15 +34,16 @@ int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
 int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
 
 /* Autotune the sweeps after Factorize: level-scheduled rows with 1..64
- * lanes per row and block-local sweeps (4096/8192/16384-row blocks in LDS)
- * with 4..32 lanes are each timed over `reps` applies; the fastest shape
- * per direction is kept.  table (optional, cap rows x 5 doubles): mode,
+ * lanes per row and the block-local sweeps (4096/8192/16384-row blocks in
+ * LDS; CSR, packed and pipelined packed factors) with 2..32 lanes are each
+ * timed over `reps` applies; the fastest shape per direction is kept.  table (optional, cap rows x 5 doubles): mode,
  * block rows, lanes, forward ms, backward ms.  Returns the rows timed. */
 #define IluSolverAutotuneEx LLJBASH_DECL(IluSolverAutotuneEx)
 int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap);
 
 /* Explicit sweep algorithm per direction (0 level rows, 1 block-local with
- * blocks of blk_rows rows) and the current configuration
+ * blocks of blk_rows rows, 2 block-local over packed factors, 3 the same
+ * software-pipelined) and the current configuration
  * (cfg[5]: mode_l, lanes_l, mode_u, lanes_u, blk_rows). */
 #define IluSolverSetSweepModeEx LLJBASH_DECL(IluSolverSetSweepModeEx)
 int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_rows);

@@ -43,6 +43,9 @@ struct SweepPack {
     int blk = 0;                    // block rows the pack was built for
     DevBuf<int> pass_ptr;
     DevBuf<char> passes;            // PackPass records
+    DevBuf<char> pdesc;             // PassDesc records (same passes, compact, padded per block)
+    DevBuf<int> desc_ptr;           // first PassDesc of each block
+    int max_np = 0;                 // most PassDesc records of one block
     int pass_g = 0;                 // lanes per row the pass list was built for
 };
 }  // namespace pilu
@@ -74,7 +77,7 @@ struct lljbash::IluSolver::Ext {
     int lanes_l = 0, lanes_u = 0;  // 0 = heuristic
     // Sweep algorithm per direction: 0 = level-scheduled rows (k_sweep),
     // 1 = block-local (k_sweep_blk) with blocks of blk_rows rows,
-    // 2 = packed block-local (k_sweep_pk).
+    // 2 = packed block-local (k_sweep_pk), 3 = pipelined packed (k_sweep_pp).
     int mode_l = 0, mode_u = 0;
     pilu::SweepPack pack_l, pack_u;
     int blk_rows = 0, nblk = 0;

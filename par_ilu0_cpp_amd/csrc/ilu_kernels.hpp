@@ -395,6 +395,204 @@ __global__ __launch_bounds__(512) void k_sweep_pk(int n, int blk, int nblk, cons
     retire_ticket(ctr);
 }
 
+// ---------------------------------------------------------------------
+// Pipelined packed sweep.  Same blocks, pack and summation order as
+// k_sweep_pk, reorganised so that no pass waits on memory it could have
+// requested earlier:
+//   * at block start the right-hand side of the block's rows is staged in
+//     LDS (xs[] then holds b_i until row i is solved and x_i afterwards;
+//     this also re-arms the companion buffer for the block's rows in bulk),
+//     and so are the block's compact pass descriptors (PassDesc);
+//   * the loads of pass t+DA (row info, column indices) and of pass t+DB
+//     (values, 1/diagonal, and a first read of every x_c outside the block)
+//     are issued right after pass t computes, in the order they will be
+//     consumed, so the in-order vector-memory counter never makes a pass
+//     wait for a younger request; all addresses are clamped into the pack
+//     so the issue sequence is branch-free;
+//   * an out-of-block x_c read early that was still pending is polled again
+//     when its pass runs.
+// Every entry is accumulated in the same order as in k_sweep / k_sweep_pk
+// with G lanes per row, so the results are bit-identical to theirs.
+// ---------------------------------------------------------------------
+// Passes per unrolled step of k_sweep_pp; every block's descriptor list is
+// padded to a multiple of it with empty passes (cb = 0: no rows).
+constexpr int kPpUnroll = 8;
+
+struct PassDesc {
+    int k_lo;   // schedule position of the pass's first row
+    int base;   // slab offset of the level group
+    int cb;     // bits 0..30: rows in the group (slab column stride); bit 31: level boundary follows
+    int jw;     // bits 0..15: index of the pass's first row within its group; bits 16..31: group width
+};
+
+template <int G, int S>
+struct PpA {
+    int row, cnt, jj;
+    bool act;
+    int c[S];
+};
+
+template <int S, typename T>
+struct PpB {
+    T v[S];
+    T dv;
+    uint64_t xg[S];
+};
+
+template <int G, int S>
+__device__ __forceinline__ void pp_issue_a(const PassDesc& d, int gid, int t, int per,
+                                           const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
+                                           PpA<G, S>& a) {
+    const int count = d.cb & 0x7fffffff;
+    const int j0 = d.jw & 0xffff;
+    const int width = d.jw >> 16;
+    const int nact = count - j0 < per ? count - j0 : per;
+    a.act = gid < nact;
+    a.jj = j0 + (a.act ? gid : 0);
+    const int2 ri = rowinfo[d.k_lo + (a.jj - j0)];
+    a.row = ri.x;
+    a.cnt = ri.y;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int q = t + s * G;
+        a.c[s] = pcol[q < width ? d.base + q * count + a.jj : 0];
+    }
+}
+
+template <int G, int S, typename T, bool UPPER>
+__device__ __forceinline__ void pp_issue_b(const PassDesc& d, int t, int bs, int be, const PpA<G, S>& a,
+                                           const T* __restrict__ pval, const T* __restrict__ pdiag,
+                                           const double* out, PpB<S, T>& b) {
+    const int count = d.cb & 0x7fffffff;
+    const int width = d.jw >> 16;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int q = t + s * G;
+        b.v[s] = pval[q < width ? d.base + q * count + a.jj : 0];
+    }
+    if (UPPER) b.dv = pdiag[d.k_lo + (a.jj - (d.jw & 0xffff))];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int q = t + s * G;
+        const int c = a.c[s];
+        // lanes with nothing to read early all read x[bs] (one request)
+        const bool need = a.act && q < a.cnt && (c < bs || c >= be);
+        int idx = need ? c : bs;
+        asm volatile("" : "+v"(idx));  // keep the read unconditional (a select, not a branch)
+        b.xg[s] = peek_bits(out + idx);
+    }
+}
+
+template <int G, int S, int DA, int DB, typename T, bool UPPER>
+__global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, const int* __restrict__ pass_ptr,
+                                                  const PassDesc* __restrict__ desc,
+                                                  const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
+                                                  const T* __restrict__ pval, const T* __restrict__ pdiag,
+                                                  const double* b, double* out, double* reset, const int* gate,
+                                                  int* ctr, int* err, uint64_t wd_limit) {
+    static_assert(DA % DB == 0 && DA > DB && DA == kPpUnroll, "ring sizes");
+    extern __shared__ double xs[];
+    if (gate && *gate == 0) return;
+    const int ticket = take_ticket(ctr);
+    const Watchdog wd(wd_limit);
+    const int bid = UPPER ? nblk - 1 - ticket : ticket;
+    const int bs = bid * blk;
+    const int be = bs + blk < n ? bs + blk : n;
+    PassDesc* sd = reinterpret_cast<PassDesc*>(xs + blk);
+    const int p0 = pass_ptr[ticket], np = pass_ptr[ticket + 1] - p0;
+    for (int i = threadIdx.x; i < be - bs; i += 512) {
+        xs[i] = b[bs + i];
+        if (reset) reset[bs + i] = __longlong_as_double(kPending);
+    }
+    for (int i = threadIdx.x; i < np; i += 512) sd[i] = desc[p0 + i];
+    __syncthreads();
+    if (np == 0) {
+        retire_ticket(ctr);
+        return;
+    }
+    constexpr int per = 512 / G;
+    const int gid = threadIdx.x / G;
+    const int t = threadIdx.x & (G - 1);
+    bool timed_out = false;
+    PpA<G, S> ra[DA];
+    PpB<S, T> rb[DB];
+#pragma unroll
+    for (int u = 0; u < DA; ++u) pp_issue_a<G, S>(sd[u < np ? u : np - 1], gid, t, per, rowinfo, pcol, ra[u]);
+#pragma unroll
+    for (int u = 0; u < DB; ++u)
+        pp_issue_b<G, S, T, UPPER>(sd[u < np ? u : np - 1], t, bs, be, ra[u], pval, pdiag, out, rb[u]);
+    // np is a multiple of DA: the unrolled body below has no pass guard, so
+    // the issue sequence (and the compiler's count of outstanding requests)
+    // is the same on every path.
+    for (int p = 0; p < np; p += DA) {
+#pragma unroll
+        for (int u = 0; u < DA; ++u) {
+            const int q = p + u;
+            {
+                const PassDesc d = sd[q];
+                PpA<G, S>& a = ra[u];
+                PpB<S, T>& r = rb[u % DB];
+                if (a.act) {
+                    double acc = t == 0 ? xs[a.row - bs] : 0.0;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        if (t + s * G < a.cnt) {
+                            const int c = a.c[s];
+                            double x;
+                            if (c >= bs && c < be) {
+                                x = xs[c - bs];
+                            } else {
+                                uint64_t bits = r.xg[s];
+                                while (bits == kPending) {
+                                    if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
+                                    spin_pause();
+                                    bits = peek_bits(out + c);
+                                }
+                                x = __longlong_as_double(bits);
+                            }
+                            acc = fma(-x, static_cast<double>(r.v[s]), acc);
+                        }
+                    }
+                    const int count = d.cb & 0x7fffffff;
+                    for (int qq = t + S * G; qq < a.cnt; qq += G) {
+                        const int pos = d.base + qq * count + a.jj;
+                        const int c = pcol[pos];
+                        double x;
+                        if (c >= bs && c < be) {
+                            x = xs[c - bs];
+                        } else {
+                            uint64_t bits;
+                            while ((bits = peek_bits(out + c)) == kPending) {
+                                if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
+                                spin_pause();
+                            }
+                            x = __longlong_as_double(bits);
+                        }
+                        acc = fma(-x, static_cast<double>(pval[pos]), acc);
+                    }
+                    const double sum = group_sum<G>(acc);
+                    if (t == 0) {
+                        const double res = UPPER ? sum / static_cast<double>(r.dv) : sum;
+                        xs[a.row - bs] = res;
+                        publish(out + a.row, res);
+                    }
+                }
+                if (d.cb < 0) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                    __builtin_amdgcn_s_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                }
+                const int qa = q + DA < np ? q + DA : np - 1;
+                pp_issue_a<G, S>(sd[qa], gid, t, per, rowinfo, pcol, ra[u]);
+                const int qb = q + DB < np ? q + DB : np - 1;
+                pp_issue_b<G, S, T, UPPER>(sd[qb], t, bs, be, ra[(u + DB) % DA], pval, pdiag, out, rb[u % DB]);
+            }
+        }
+    }
+    if (timed_out) atomicOr(err, kErrTimeout);
+    retire_ticket(ctr);
+}
+
 // Values of the packed sweep from the factors: pval[e] = fac[src[e]] (src
 // < 0: padding), pdiag[k] = fac[dg[row of k]].
 template <typename TI, typename TO>

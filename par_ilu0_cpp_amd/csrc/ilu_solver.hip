@@ -307,6 +307,41 @@ void build_passes(Ext* e, SweepPack& pk, int G) {
         }
     }
     ptr[e->nblk] = static_cast<int>(passes.size());
+    // Compact descriptors for the pipelined sweep, each block's list padded
+    // with empty passes to a multiple of kPpUnroll.
+    std::vector<PassDesc> desc;
+    std::vector<int> dptr(static_cast<size_t>(e->nblk) + 1, 0);
+    pk.max_np = 0;
+    for (int p = 0; p < e->nblk; ++p) {
+        dptr[p] = static_cast<int>(desc.size());
+        for (int i = ptr[p]; i < ptr[p + 1]; ++i) {
+            const PackPass& r = passes[i];
+            const int j0 = r.k_lo - r.gstart;
+            if (j0 >= (1 << 16) || r.width >= (1 << 15))
+                throw std::runtime_error("packed sweep: level group too large for the pass descriptor");
+            // every slab position and schedule position the pass can touch
+            // lies inside the pack
+            const int nact = std::min(512 / G, r.count - j0);
+            if (nact <= 0 || r.k_lo + nact > e->n ||
+                (r.width > 0 && r.base + static_cast<long>(r.width - 1) * r.count + j0 + nact > std::max<long>(pk.ne, 1)))
+                throw std::runtime_error("packed sweep: pass descriptor out of range");
+            PassDesc d;
+            d.k_lo = r.k_lo;
+            d.base = r.base;
+            d.cb = r.count | (r.bar ? static_cast<int>(0x80000000u) : 0);
+            d.jw = j0 | (r.width << 16);
+            desc.push_back(d);
+        }
+        const int k_any = ptr[p + 1] > ptr[p] ? passes[ptr[p]].k_lo : 0;
+        while ((desc.size() - dptr[p]) % kPpUnroll) desc.push_back(PassDesc{k_any, 0, 0, 0});
+        pk.max_np = std::max(pk.max_np, static_cast<int>(desc.size()) - dptr[p]);
+    }
+    dptr[e->nblk] = static_cast<int>(desc.size());
+    if (desc.empty()) desc.push_back(PassDesc{0, 0, 0, 0});
+    pk.pdesc.resize(sizeof(PassDesc) * desc.size());
+    pk.desc_ptr.resize(dptr.size());
+    PILU_HIP(hipMemcpy(pk.pdesc.get(), desc.data(), sizeof(PassDesc) * desc.size(), hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(pk.desc_ptr.get(), dptr.data(), sizeof(int) * dptr.size(), hipMemcpyHostToDevice));
     pk.pass_ptr.resize(ptr.size());
     pk.passes.resize(sizeof(PackPass) * std::max<size_t>(passes.size(), 1));
     PILU_HIP(hipMemcpy(pk.pass_ptr.get(), ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
@@ -409,6 +444,38 @@ void sweep_pk_one(Ext* e, const T* pval, const T* pdiag, const double* b, double
                                           watchdog_ticks(kWatchdogSeconds));
 }
 
+// LDS of the pipelined packed sweep: the block's x (b on entry) and its
+// pass descriptors.
+size_t pp_lds_bytes(const Ext* e, const SweepPack& pk) {
+    return sizeof(double) * static_cast<size_t>(e->blk_rows) + sizeof(PassDesc) * static_cast<size_t>(pk.max_np);
+}
+constexpr size_t kMaxLds = 160 * 1024;
+
+// True when the pipelined packed sweep with G lanes per row fits the LDS in
+// both directions for the current blocks.
+bool pp_fits(Ext* e, int G) {
+    build_passes(e, e->pack_l, G);
+    build_passes(e, e->pack_u, G);
+    return pp_lds_bytes(e, e->pack_l) <= kMaxLds && pp_lds_bytes(e, e->pack_u) <= kMaxLds;
+}
+
+template <int G, int S, typename T, bool UPPER>
+void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double* out, double* reset, const int* gate,
+                  int* ctr) {
+    SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
+    build_passes(e, pk, G);
+    const size_t lds = pp_lds_bytes(e, pk);
+    if (lds > kMaxLds) throw std::runtime_error("pipelined packed sweep: block and pass list exceed the LDS");
+    auto kern = k_sweep_pp<G, S, kPpUnroll, kPpUnroll / 2, T, UPPER>;
+    PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(lds)));
+    kern<<<e->nblk, 512, lds, e->stream>>>(e->n, e->blk_rows, e->nblk, pk.desc_ptr.get(),
+                                          reinterpret_cast<const PassDesc*>(pk.pdesc.get()),
+                                          reinterpret_cast<const int2*>(pk.rowinfo.get()), pk.pcol.get(), pval, pdiag,
+                                          b, out, reset, gate, ctr, e->err.get() + kErrSweep,
+                                          watchdog_ticks(kWatchdogSeconds));
+}
+
 template <int G, typename T, bool UPPER>
 void sweep_pk_s(Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
     SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
@@ -422,6 +489,13 @@ void sweep_pk_s(Ext* e, const double* b, double* out, double* reset, const int* 
         pdiag = UPPER ? pk.pdiag64.get() : nullptr;
     }
     const int need = (pk.max_w + G - 1) / G;
+    const bool pipelined = (UPPER ? e->mode_u : e->mode_l) == 3;
+    if (pipelined) {
+        if (need <= 1) sweep_pp_one<G, 1, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+        else if (need <= 2) sweep_pp_one<G, 2, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+        else sweep_pp_one<G, 4, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+        return;
+    }
     if (need <= 1) sweep_pk_one<G, 1, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
     else if (need <= 2) sweep_pk_one<G, 2, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
     else sweep_pk_one<G, 4, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
@@ -441,7 +515,7 @@ void sweep_pk_dispatch(int lanes, Ext* e, const double* b, double* out, double* 
 template <typename T, bool UPPER>
 void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
                     const int* gate, int* ctr) {
-    if ((UPPER ? e->mode_u : e->mode_l) == 2 && e->blk_rows > 0) {
+    if ((UPPER ? e->mode_u : e->mode_l) >= 2 && e->blk_rows > 0) {
         sweep_pk_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
         return;
     }
@@ -750,6 +824,15 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
             if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 2; }
             if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 2; }
         }
+        e->mode_l = e->mode_u = 3;
+        for (int g : kPackLanes) {
+            double tl, tu;
+            if (!pilu::pp_fits(e, g)) continue;
+            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+            record(3, blk, g, tl, tu);
+            if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 3; }
+            if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 3; }
+        }
         const double tot = std::min(bb[0], lev_best[0]) + std::min(bb[1], lev_best[1]);
         if (tot < best_total) {
             best_total = tot;
@@ -763,7 +846,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     for (auto& v : ev) (void)hipEventDestroy(v);
     pilu::check_sweep_errors(e);
     if (pick_blk > 0) pilu::use_blocks(e, pick_blk);
-    if (pick_mode[0] == 2 || pick_mode[1] == 2) pilu::ensure_packs(e);
+    if (pick_mode[0] >= 2 || pick_mode[1] >= 2) pilu::ensure_packs(e);
     e->mode_l = pick_mode[0];
     e->mode_u = pick_mode[1];
     e->lanes_l = pick_lanes[0];
@@ -773,9 +856,9 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
 
 void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
     Ext* e = pilu::need_setup(ext_);
-    if (mode_l < 0 || mode_l > 2 || mode_u < 0 || mode_u > 2) throw std::runtime_error("sweep mode is 0, 1 or 2");
+    if (mode_l < 0 || mode_l > 3 || mode_u < 0 || mode_u > 3) throw std::runtime_error("sweep mode is 0, 1, 2 or 3");
     if (mode_l > 0 || mode_u > 0) pilu::use_blocks(e, blk_rows);
-    if (mode_l == 2 || mode_u == 2) pilu::ensure_packs(e);
+    if (mode_l >= 2 || mode_u >= 2) pilu::ensure_packs(e);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
 }

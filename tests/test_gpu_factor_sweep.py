@@ -168,7 +168,7 @@ def test_autotune_keeps_results(oracle):
     assert np.max(np.abs(x1 - x0)) <= 1e-12 * np.max(np.abs(x0))
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("blk", [64, 1000, 4096, 100000])
 @pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_block_sweep_bit_identical_to_level_sweep(oracle, mode, blk, lanes):
@@ -197,7 +197,7 @@ def test_block_sweep_bit_identical_to_level_sweep(oracle, mode, blk, lanes):
 def test_packed_sweep_follows_refactorization(oracle):
     a = gen.convdiff3d(12)
     ilu = factored(a.row_ptr, a.col_idx, a.value)
-    ilu.SetSweepMode(2, 2, 512)
+    ilu.SetSweepMode(2, 3, 512)
     va2 = a.value * np.linspace(1.0, 1.5, a.nnz)
     P.fill_csr(ilu.GetMatrix(), a.row_ptr, a.col_idx, va2)
     assert ilu.Factorize() == 0
@@ -215,7 +215,7 @@ def test_block_sweep_random_pattern(oracle):
     b = np.random.default_rng(2).uniform(-1, 1, 5000)
     xo = oracle.substitute(rp, ci, dg, ofac, b)
     ilu.SetSweepLanes(4, 4)
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         ilu.SetSweepMode(mode, mode, 512)
         x = ilu.Substitute(b)
         assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
