@@ -1,0 +1,256 @@
+// Command-line benchmark over the C++ API: reads a MatrixMarket matrix,
+// builds ILU(k) with IlukSymbolic, and times SetupMatrix / Factorize /
+// GMRES (or only Factorize, or only Substitute) with x_true = 1, b = A x_true.
+//
+// Same options, phase protocol and summary lines as the reference's
+// test/benchmark.cpp:16-213 (value refresh through nzmap before every
+// Factorize, SetupMatrix on the first run only, warm-ups excluded from the
+// totals), so existing scripts that parse its output keep working.  Extra
+// options drive this build's extensions: --fp32 stores the factors in FP32,
+// --autotune picks the sweep launch shape after the first factorization.
+// Timers are wall-clock (steady_clock); the GPU work is synchronous at each
+// phase boundary.
+#include <getopt.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "gmres.hpp"
+#include "ilu_solver.hpp"
+#include "iluk.h"
+
+using namespace lljbash;
+
+namespace {
+
+struct Options {
+    std::string mat;
+    int lof = 0;
+    bool amd = false, nd = false;
+    double eps = 1e-8;
+    int maxit = 1000;
+    int restart = 60;
+    int threads = 1;
+    int rep = 1;
+    int warmup = 0;
+    bool only_fact = false, only_subs = false;
+    bool fp32 = false, autotune = false;
+};
+
+void usage(const char* prog) {
+    std::printf(
+        "benchmark for ILU(k) GMRES on the GPU\n"
+        "Usage:\n  %s [OPTION...]\n\n"
+        "  -m, --mat arg      Coefficient matrix file (MM1)\n"
+        "  -k, --lof arg      ILU(k) level of fill (default: 0)\n"
+        "      --amd          AMD reorder\n"
+        "      --nd           ND reorder\n"
+        "  -e, --eps arg      GMRES tolerance (default: 1e-8)\n"
+        "  -i, --maxit arg    GMRES max iterations (default: 1000)\n"
+        "  -r, --restart arg  GMRES restart iterations (default: 60)\n"
+        "  -t, --threads arg  Number of threads (default: 1; unused on the GPU)\n"
+        "  -n, --rep arg      Number of repeats (default: 1)\n"
+        "  -w, --warmup [=N]  Include warmups (default: 1 when given)\n"
+        "      --onlyfact     only test factorization\n"
+        "      --onlysubs     only test substitution\n"
+        "      --fp32         FP32 factors in the sweeps (FP64 everywhere else)\n"
+        "      --autotune     pick the sweep launch shape after the first factorization\n"
+        "  -h, --help         Print usage\n",
+        prog);
+}
+
+Options parse(int argc, char** argv) {
+    enum { kAmd = 1000, kNd, kOnlyFact, kOnlySubs, kFp32, kAutotune };
+    static const option longopts[] = {
+        {"mat", required_argument, nullptr, 'm'},      {"lof", required_argument, nullptr, 'k'},
+        {"amd", no_argument, nullptr, kAmd},           {"nd", no_argument, nullptr, kNd},
+        {"eps", required_argument, nullptr, 'e'},      {"maxit", required_argument, nullptr, 'i'},
+        {"restart", required_argument, nullptr, 'r'},  {"threads", required_argument, nullptr, 't'},
+        {"rep", required_argument, nullptr, 'n'},      {"warmup", optional_argument, nullptr, 'w'},
+        {"onlyfact", no_argument, nullptr, kOnlyFact}, {"onlysubs", no_argument, nullptr, kOnlySubs},
+        {"fp32", no_argument, nullptr, kFp32},         {"autotune", no_argument, nullptr, kAutotune},
+        {"help", no_argument, nullptr, 'h'},           {nullptr, 0, nullptr, 0}};
+    Options o;
+    int c;
+    while ((c = getopt_long(argc, argv, "m:k:e:i:r:t:n:w::h", longopts, nullptr)) != -1) {
+        switch (c) {
+            case 'm': o.mat = optarg; break;
+            case 'k': o.lof = std::atoi(optarg); break;
+            case kAmd: o.amd = true; break;
+            case kNd: o.nd = true; break;
+            case 'e': o.eps = std::atof(optarg); break;
+            case 'i': o.maxit = std::atoi(optarg); break;
+            case 'r': o.restart = std::atoi(optarg); break;
+            case 't': o.threads = std::atoi(optarg); break;
+            case 'n': o.rep = std::atoi(optarg); break;
+            case 'w': o.warmup = optarg ? std::atoi(optarg) : 1; break;
+            case kOnlyFact: o.only_fact = true; break;
+            case kOnlySubs: o.only_subs = true; break;
+            case kFp32: o.fp32 = true; break;
+            case kAutotune: o.autotune = true; break;
+            case 'h': usage(argv[0]); std::exit(0);
+            default: usage(argv[0]); std::exit(1);
+        }
+    }
+    if (o.mat.empty()) {
+        std::fprintf(stderr, "error: --mat is required\n");
+        std::exit(1);
+    }
+    return o;
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Symmetric permutation of the matrix rows and columns (new index ip[old]),
+// columns re-sorted within each row.
+void permute(CsrMatrix* a, const std::vector<int>& ip) {
+    const int n = a->size;
+    std::vector<std::vector<std::pair<int, double>>> rows(n);
+    for (int i = 0; i < n; ++i)
+        for (int p = a->row_ptr[i]; p < a->row_ptr[i + 1]; ++p)
+            rows[ip[i]].emplace_back(ip[a->col_idx[p]], a->value[p]);
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        std::sort(rows[i].begin(), rows[i].end());
+        a->row_ptr[i] = k;
+        for (auto& [j, v] : rows[i]) {
+            a->col_idx[k] = j;
+            a->value[k] = v;
+            ++k;
+        }
+    }
+    a->row_ptr[n] = k;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const Options opt = parse(argc, argv);
+    try {
+        CsrMatrix csr = CSR_MATRIX_DEFAULT;
+        ReadCsrMatrixMM1(&csr, opt.mat.c_str());
+        const int n = csr.size;
+        const int nnz = GetCsrNonzeros(&csr);
+        if (n <= 0) {
+            std::fprintf(stderr, "error: could not read %s\n", opt.mat.c_str());
+            return 1;
+        }
+        if (opt.amd || opt.nd) {
+            std::vector<int> p(n), ip(n);
+            const int rc = opt.amd ? CsrAmdOrder(&csr, p.data(), ip.data()) : CsrNdOrder(&csr, p.data(), ip.data());
+            if (rc != 0) {
+                std::fprintf(stderr, "error: %s ordering is not available in this build\n", opt.amd ? "AMD" : "ND");
+                return 1;
+            }
+            permute(&csr, ip);
+        }
+
+        std::vector<double> rhs(n), sol(n, 1.0), x(n);
+        CsrMatVec(&csr, sol.data(), rhs.data());
+
+        IluSolver ilu;
+        CsrMatrix* f = ilu.GetMatrix();
+        PreconditionedGmres gmres;
+        gmres.SetIluPreconditioner(&ilu);
+        gmres.param().tolerance = opt.eps;
+        gmres.param().max_iterations = opt.maxit;
+        gmres.param().krylov_subspace_dimension = opt.restart;
+        ilu.SetThreads(opt.threads);
+        if (opt.fp32) ilu.SetFactorPrecision(32);
+
+        std::vector<int> nzmap(nnz);
+        const int knnz = IlukSymbolic(&csr, opt.lof, f, nzmap.data());
+        if (knnz < 0) {
+            std::fprintf(stderr, "error: IlukSymbolic failed\n");
+            return 1;
+        }
+        std::printf("knnz: %d\n", knnz);
+
+        double setup_time = 0.0, fact_time = 0.0, gmres_time = 0.0, subs_time = 0.0;
+        bool tuned = false;
+        auto timed = [](const std::function<void()>& fn, double& acc) {
+            const auto t0 = std::chrono::steady_clock::now();
+            fn();
+            acc += seconds_since(t0);
+        };
+        auto refresh = [&]() {
+            std::memset(f->value, 0, sizeof(double) * knnz);
+            for (int i = 0; i < nnz; ++i) f->value[nzmap[i]] = csr.value[i];
+        };
+        auto factor = [&](bool setup) {
+            refresh();
+            if (setup) timed([&] { ilu.SetupMatrix(); }, setup_time);
+            bool ok = true;
+            timed([&] { ok = ilu.Factorize(); }, fact_time);
+            if (!ok) std::puts("warning: zero pivot in the factorization");
+            if (opt.autotune && !tuned) {
+                ilu.AutotuneSweeps(3, nullptr, 0);
+                tuned = true;
+            }
+        };
+        std::function<void(bool)> run;
+        if (opt.only_fact) {
+            run = factor;
+        } else if (opt.only_subs) {
+            run = [&](bool setup) {
+                if (setup) factor(true);
+                timed([&] { ilu.Substitute(rhs.data(), x.data()); }, subs_time);
+            };
+        } else {
+            run = [&](bool setup) {
+                factor(setup);
+                std::fill(x.begin(), x.end(), 0.0);
+                std::pair<bool, int> ret;
+                timed([&] { ret = gmres.Solve(&csr, rhs.data(), x.data()); }, gmres_time);
+                if (!ret.first) {
+                    std::puts("GMRES error");
+                    std::exit(0);
+                }
+                std::printf("iter: %d\n", ret.second);
+            };
+        }
+        for (int i = 0; i < opt.warmup; ++i) run(i == 0);
+        setup_time = fact_time = gmres_time = 0.0;
+        const GmresStat warm = gmres.stat();
+        for (int i = 0; i < opt.rep; ++i) run(i == 0);
+        const double mv_time = gmres.stat().total_mv_time - warm.total_mv_time;
+        const double precon_time = opt.only_subs ? subs_time : gmres.stat().total_precon_time - warm.total_precon_time;
+
+        double max_err = 0.0;
+        for (int i = 0; i < n; ++i) max_err = std::max(max_err, std::abs(x[i] - sol[i]));
+        std::printf("max_err: %g\n", max_err);
+
+        std::puts("\nSummary:");
+        std::printf("n:                          %d\n", n);
+        std::printf("nnz:                        %d\n", nnz);
+        std::printf("k:                          %d\n", opt.lof);
+        std::printf("nnz(F)/nnz(A):              %f\n", static_cast<double>(knnz) / nnz);
+        std::printf("gmres tolerance:            %g\n", gmres.param().tolerance);
+        std::printf("gmres max iterations:       %d\n", gmres.param().max_iterations);
+        std::printf("gmres restart iterations:   %d\n", gmres.param().krylov_subspace_dimension);
+        std::printf("threads:                    %d\n", opt.threads);
+        std::printf("reps:                       %d\n", opt.rep);
+        std::printf("setup time (s):             %g\n", setup_time);
+        std::printf("total factorize time (s):   %g\n", fact_time);
+        std::printf("total gmres time (s):       %g\n", gmres_time);
+        std::printf("total mv time (s):          %g\n", mv_time);
+        std::printf("total precon time (s):      %g\n", precon_time);
+        std::printf("average factorize time (s): %g\n", fact_time / opt.rep);
+        std::printf("average precon time (s):    %g\n", precon_time / opt.rep);
+        std::printf("average gmres time (s):     %g\n", gmres_time / opt.rep);
+        DestroyCsrMatrix(&csr);
+    } catch (const std::exception& ex) {
+        std::fprintf(stderr, "error: %s\n", ex.what());
+        return 1;
+    }
+    return 0;
+}
