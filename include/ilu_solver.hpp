@@ -101,6 +101,13 @@ public:
     // Copy the level of every entry of F (from the last SymbolicDevice) to
     // the host; returns nnz(F).
     int CopyFillLevels(signed char* out) const;
+    // The schedules and sweep configuration as one JSON object: levels per
+    // direction, block rows and blocks, sweep mode and lanes per direction,
+    // packed entries (with padding) and passes.  Printed to stdout after
+    // SetupMatrix and after a sweep-shape change when the environment
+    // variable PAR_ILU_SHOW_ALGORITHM is set (the reference prints its
+    // schedule under the SHOW_ALGORITHM macro, src/ilu_solver.cpp:196-223).
+    std::string DescribeSchedule() const;
     int Size() const;
     int Nonzeros() const;
 

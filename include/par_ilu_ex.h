@@ -51,6 +51,11 @@ int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_ro
 int IluSolverGetSweepConfigEx(IluSolverHdl hdl, int* cfg);
 
 /* Number of levels of the L and U dependency graphs (after setup). */
+/* IluSolver::DescribeSchedule as a NUL-terminated JSON string in buf
+ * (truncated to cap - 1 characters); returns the full length, -1 on error. */
+#define IluSolverScheduleJsonEx LLJBASH_DECL(IluSolverScheduleJsonEx)
+int IluSolverScheduleJsonEx(IluSolverHdl hdl, char* buf, int cap);
+
 #define IluSolverGetLevelsEx LLJBASH_DECL(IluSolverGetLevelsEx)
 int IluSolverGetLevelsEx(IluSolverHdl hdl, int* nlev_l, int* nlev_u);
 

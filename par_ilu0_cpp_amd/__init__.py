@@ -10,6 +10,7 @@ test/benchmark.cpp).  All numerical work runs in the HIP libraries built from
 from __future__ import annotations
 
 import ctypes as C
+import json
 
 import numpy as np
 
@@ -192,6 +193,15 @@ class IluSolver:
     def SetSweepMode(self, mode_l: int, mode_u: int, blk_rows: int = 0) -> None:
         if fn("IluSolverSetSweepModeEx")(self.hdl, int(mode_l), int(mode_u), int(blk_rows)) != 0:
             raise RuntimeError("SetSweepMode failed")
+
+    def Schedule(self) -> dict:
+        """Schedules and sweep configuration (IluSolver::DescribeSchedule)."""
+        need = fn("IluSolverScheduleJsonEx")(self.hdl, None, 0)
+        if need < 0:
+            raise RuntimeError("IluSolverScheduleJsonEx failed")
+        buf = C.create_string_buffer(need + 1)
+        fn("IluSolverScheduleJsonEx")(self.hdl, buf, need + 1)
+        return json.loads(buf.value.decode())
 
     def SweepConfig(self) -> dict:
         cfg = np.zeros(5, dtype=np.int32)

@@ -85,6 +85,7 @@ SIGNATURES = [
     ("IluSolverSetFactorPrecisionEx", _int, [_vp, _int]),
     ("IluSolverSetSweepLanesEx", _int, [_vp, _int, _int]),
     ("IluSolverGetLevelsEx", _int, [_vp, P(_int), P(_int)]),
+    ("IluSolverScheduleJsonEx", _int, [_vp, C.c_char_p, _int]),
     ("IluSolverAutotuneEx", _int, [_vp, _int, P(_dbl), _int]),
     ("IluSolverSetSweepModeEx", _int, [_vp, _int, _int, _int]),
     ("IluSolverGetSweepConfigEx", _int, [_vp, P(_int)]),

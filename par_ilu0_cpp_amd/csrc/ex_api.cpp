@@ -57,6 +57,18 @@ int IluSolverGetSweepConfigEx(IluSolverHdl hdl, int* cfg) {
     return 0;
 }
 
+int IluSolverScheduleJsonEx(IluSolverHdl hdl, char* buf, int cap) {
+    return guarded("ScheduleJson", [&] {
+        const std::string js = S(hdl)->DescribeSchedule();
+        if (buf && cap > 0) {
+            const size_t k = std::min(js.size(), static_cast<size_t>(cap - 1));
+            std::memcpy(buf, js.data(), k);
+            buf[k] = '\0';
+        }
+        return static_cast<int>(js.size());
+    });
+}
+
 int IluSolverGetLevelsEx(IluSolverHdl hdl, int* nlev_l, int* nlev_u) {
     if (nlev_l) *nlev_l = S(hdl)->NumLevelsL();
     if (nlev_u) *nlev_u = S(hdl)->NumLevelsU();

@@ -6,7 +6,9 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <stdexcept>
 #include <vector>
 
@@ -583,6 +585,12 @@ Ext* need_setup(Ext* e) {
 
 }  // namespace
 
+// Prints the schedule description when PAR_ILU_SHOW_ALGORITHM is set.
+void show_algorithm(const lljbash::IluSolver* solver) {
+    const char* v = std::getenv("PAR_ILU_SHOW_ALGORITHM");
+    if (v && *v && std::strcmp(v, "0") != 0) std::printf("%s\n", solver->DescribeSchedule().c_str());
+}
+
 void ilu_apply(Ext* e, const double* d_b, double* d_x, const int* gate) {
     need_setup(e);
     if (!e->factored) throw std::runtime_error("IluSolver: Factorize has not been called");
@@ -632,6 +640,7 @@ void IluSolver::SetupMatrix() {
     if (nnz > 0)
         PILU_HIP(hipMemcpy(e->ci.get(), aMatrix_.col_idx, sizeof(int) * nnz, hipMemcpyHostToDevice));
     pilu::analyse(e, aMatrix_.row_ptr);
+    pilu::show_algorithm(this);
 }
 
 void IluSolver::SetupMatrixDevice(int n, int nnz, const int* d_row_ptr, const int* d_col_idx) {
@@ -688,6 +697,7 @@ int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx,
     e->nzmap = std::move(res.nzmap);
     e->fill_level = std::move(res.lev);
     pilu::analyse(e, nullptr);
+    pilu::show_algorithm(this);
     return nnz_f;
 }
 
@@ -851,6 +861,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     e->mode_u = pick_mode[1];
     e->lanes_l = pick_lanes[0];
     e->lanes_u = pick_lanes[1];
+    pilu::show_algorithm(this);
     return rows;
 }
 
@@ -861,6 +872,29 @@ void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
     if (mode_l >= 2 || mode_u >= 2) pilu::ensure_packs(e);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
+    pilu::show_algorithm(this);
+}
+
+std::string IluSolver::DescribeSchedule() const {
+    const Ext* e = ext_;
+    if (!e || e->n <= 0) return "{}";
+    char buf[1024];
+    auto pack = [](const pilu::SweepPack& pk) {
+        char b[256];
+        std::snprintf(b, sizeof(b), "{\"entries\": %ld, \"max_width\": %d, \"lanes\": %d, \"max_passes_per_block\": %d}",
+                      pk.ne, pk.max_w, pk.pass_g, pk.max_np);
+        return std::string(b);
+    };
+    std::snprintf(buf, sizeof(buf),
+                  "{\"n\": %d, \"nnz\": %ld, \"max_row\": %d, \"precision\": %d, \"levels_l\": %d, "
+                  "\"levels_u\": %d, \"blk_rows\": %d, \"blocks\": %d, \"mode_l\": %d, \"lanes_l\": %d, "
+                  "\"mode_u\": %d, \"lanes_u\": %d, ",
+                  e->n, e->nnz, e->max_row, e->precision, e->nlev_l, e->nlev_u, e->blk_rows, e->nblk, e->mode_l,
+                  pilu::effective_lanes(e, false), e->mode_u, pilu::effective_lanes(e, true));
+    std::string js(buf);
+    js += "\"pack_l\": " + (e->pack_l.blk ? pack(e->pack_l) : std::string("null"));
+    js += ", \"pack_u\": " + (e->pack_u.blk ? pack(e->pack_u) : std::string("null")) + "}";
+    return js;
 }
 
 void IluSolver::GetSweepConfig(int* cfg) const {

@@ -219,3 +219,24 @@ def test_block_sweep_random_pattern(oracle):
         ilu.SetSweepMode(mode, mode, 512)
         x = ilu.Substitute(b)
         assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+
+
+@pytest.mark.parametrize("lof,closed_form", [(0, lambda s: 3 * s - 2), (1, lambda s: 6 * s - 5)])
+def test_schedule_levels_closed_form(oracle, lof, closed_form):
+    """Levels of the L and U dependency graphs of a 3D 7-point stencil:
+    3s-2 for ILU(0) and 6s-5 for ILU(1) (the fill couples (x+1, y-1) and
+    similar diagonals, doubling the critical path)."""
+    s = 11
+    a = gen.convdiff3d(s)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, lof)
+    ilu = factored(o[0], o[1], o[3])
+    sch = ilu.Schedule()
+    assert sch["n"] == a.n and sch["nnz"] == o[0][-1]
+    assert sch["levels_l"] == closed_form(s) and sch["levels_u"] == closed_form(s)
+    assert sch["pack_l"] is None
+    ilu.SetSweepMode(3, 3, 256)
+    b = np.ones(a.n)
+    ilu.Substitute(b)
+    sch = ilu.Schedule()
+    assert sch["mode_l"] == 3 and sch["blk_rows"] == 256 and sch["blocks"] == -(-a.n // 256)
+    assert sch["pack_l"]["entries"] >= (o[0][-1] - a.n) // 2 and sch["pack_l"]["max_passes_per_block"] % 8 == 0
