@@ -51,8 +51,32 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=12,
                    help="Arnoldi steps in the bounded CPU baseline sample (0 = skip)")
     p.add_argument("--no-autotune", action="store_true", help="keep the default sweep shape")
-    p.add_argument("--traffic", default="", help="JSON file with measured HBM bytes per apply (rocprof)")
+    p.add_argument("--sweep", default="",
+                   help="fixed sweep shape MODE_L:LANES_L:MODE_U:LANES_U:BLOCK_ROWS instead of autotuning")
+    p.add_argument("--traffic", default=str(ROOT / "profiles" / "sweep_traffic_convdiff128.txt"),
+                   help="rocprofv3 summary holding the measured HBM bytes per apply")
+    p.add_argument("--copy-calib", action="store_true",
+                   help="only run a 1 GiB device copy three times (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
+
+
+SWEEP_KERNELS = {0: "k_sweep", 1: "k_sweep_blk", 2: "k_sweep_pk", 3: "k_sweep_pp"}
+
+
+def copy_calibration():
+    """A plain device copy of known size: 2^27 doubles (1 GiB read, 1 GiB
+    written) three times.  Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its
+    counters give the factor between counted and actual HBM bytes."""
+    import torch
+
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
 
 
 def tri_bytes(n: int, nnz_f: int, value_bytes: int) -> int:
@@ -64,6 +88,9 @@ def tri_bytes(n: int, nnz_f: int, value_bytes: int) -> int:
 
 def main():
     args = parse()
+    if args.copy_calib:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -109,7 +136,14 @@ def main():
         nl, nu = ilu.Levels()
         log(f"[bench] {form}: symbolic+analysis {t_setup*1e3:.1f} ms nnz(F)={nnz_f} levels L={nl} U={nu}")
         sweep_cfg = ilu.SweepConfig()
-        if not args.no_autotune:
+        if args.sweep:
+            ml, ll, mu, lu, blk = (int(v) for v in args.sweep.split(":"))
+            ilu.SetSweepLanes(ll, lu)
+            if ml or mu:
+                ilu.SetSweepMode(ml, mu, blk)
+            sweep_cfg = ilu.SweepConfig()
+            log(f"[bench] {form}: fixed sweep shape {sweep_cfg}")
+        elif not args.no_autotune:
             # one factorization, then time every sweep shape (part of setup, untimed)
             if ilu.FactorizeDevice(d_va.data_ptr(), True) != 0:
                 raise SystemExit("factorization failed")
@@ -194,7 +228,8 @@ def main():
         "speedup_fp32_vs_fp64": (results["fp64"]["s_per_solve"] / results["fp32"]["s_per_solve"]
                                  if "fp64" in results and "fp32" in results else None),
         "roofline": {
-            "kernel": "triangular sweep apply (k_sweep forward + backward)",
+            "kernel": "triangular sweep apply: {} (L) + {} (U)".format(
+                SWEEP_KERNELS.get(r["sweep"]["mode_l"], "?"), SWEEP_KERNELS.get(r["sweep"]["mode_u"], "?")),
             "bound": "hbm",
             "achieved": r["tri_gbs"],
             "peak": HBM_PEAK_GBS,
@@ -210,13 +245,17 @@ def main():
 
 
 def traffic_bytes(path, form):
-    """HBM bytes per apply measured by rocprofv3 FETCH_SIZE + WRITE_SIZE
-    (calibrated against a copy kernel), recorded in `path` by the profiling
-    run; None when not measured."""
+    """HBM bytes per apply (L + U sweep kernels) measured by rocprofv3
+    FETCH_SIZE + WRITE_SIZE and calibrated against a copy kernel, read from
+    the profile summary's "traffic_bytes_per_apply <form> <bytes>" line;
+    None when not measured."""
     if not path or not Path(path).exists():
         return None
-    d = json.loads(Path(path).read_text())
-    return d.get(form)
+    for line in Path(path).read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_apply" and f[1] == form:
+            return float(f[2])
+    return None
 
 
 def cpu_baseline(a, b, tol, gpu_its, steps):
