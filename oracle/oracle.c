@@ -204,6 +204,34 @@ void orc_substitute(int n, const int* rp, const int* ci, const int* diag,
 }
 
 /* ------------------------------------------------------------------ */
+/* The same sweeps with each row's sum split over g lanes (g a power of  */
+/* two <= 64): lane t accumulates entries t, t+g, t+2g, ... of the row    */
+/* in order with fma, lane 0 starting from the right-hand side; then the  */
+/* lanes are combined pairwise, lane t += lane t+off for off = g/2, g/4,  */
+/* ..., 1.  This is the order of the GPU sweeps with g lanes per row; g = */
+/* 1 is orc_substitute.                                                   */
+/* ------------------------------------------------------------------ */
+static double lane_sum(int g, double init, int p0, int p1, const int* ci, const double* val, const double* x) {
+    double part[64];
+    for (int t = 0; t < g; ++t) part[t] = 0.0;
+    part[0] = init;
+    for (int p = p0; p < p1; ++p) {
+        const int t = (p - p0) % g;
+        part[t] = fma(-x[ci[p]], val[p], part[t]);
+    }
+    for (int off = g / 2; off > 0; off /= 2)
+        for (int t = 0; t < off; ++t) part[t] += part[t + off];
+    return part[0];
+}
+
+void orc_substitute_lanes(int n, const int* rp, const int* ci, const int* diag, const double* val,
+                          const double* b, double* x, int g) {
+    if (b != x) memcpy(x, b, (size_t)n * sizeof(double));
+    for (int i = 0; i < n; ++i) x[i] = lane_sum(g, x[i], rp[i], diag[i], ci, val, x);
+    for (int i = n - 1; i >= 0; --i) x[i] = lane_sum(g, x[i], diag[i] + 1, rp[i + 1], ci, val, x) / val[diag[i]];
+}
+
+/* ------------------------------------------------------------------ */
 /* y = alpha A x + beta y (the mkl_sparse_d_mv calls of src/gmres.cpp). */
 /* ------------------------------------------------------------------ */
 void orc_spmv(int n, const int* rp, const int* ci, const double* val,

@@ -67,6 +67,11 @@ int orc_ilu_factorize_comp(int n, const int* rp, const int* ci, const int* diag,
  * entries, divide by the diagonal.  b may alias x. */
 void orc_substitute(int n, const int* rp, const int* ci, const int* diag,
                     const double* val, const double* b, double* x);
+/* orc_substitute with each row's sum split over g lanes in the GPU sweeps'
+ * order (lane-strided fma chains, then a pairwise tree); g = 1 equals
+ * orc_substitute. */
+void orc_substitute_lanes(int n, const int* rp, const int* ci, const int* diag, const double* val,
+                          const double* b, double* x, int g);
 
 /* y = alpha A x + beta y, row sums accumulated in entry order. */
 void orc_spmv(int n, const int* rp, const int* ci, const double* val,

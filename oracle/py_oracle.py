@@ -36,6 +36,7 @@ def _load(fast: bool = False) -> C.CDLL:
     lib.orc_ilu_factorize_comp.restype = C.c_int
     lib.orc_ilu_factorize_comp.argtypes = [C.c_int, _I, _I, _I, _D, C.POINTER(C.c_float)]
     lib.orc_substitute.argtypes = [C.c_int, _I, _I, _I, _D, _D, _D]
+    lib.orc_substitute_lanes.argtypes = [C.c_int, _I, _I, _I, _D, _D, _D, C.c_int]
     lib.orc_spmv.argtypes = [C.c_int, _I, _I, _D, C.c_double, _D, C.c_double, _D]
     lib.orc_gmres.restype = C.c_int
     lib.orc_gmres.argtypes = [C.c_int, _I, _I, _D, _I, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int,
@@ -122,6 +123,17 @@ def substitute(row_ptr, col_idx, diag, factors, b, fast: bool = False):
     bb = _d(b)
     x = np.empty_like(bb)
     lib.orc_substitute(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(fv), _pd(bb), _pd(x))
+    return x
+
+
+def substitute_lanes(row_ptr, col_idx, diag, factors, b, lanes: int):
+    """orc_substitute with each row's sum split over `lanes` lanes in the
+    GPU sweeps' order (bit-exact reference for any lanes per row)."""
+    lib = _load()
+    rp, ci, dg, fv = _i(row_ptr), _i(col_idx), _i(diag), _d(factors)
+    bb = _d(b)
+    x = np.empty_like(bb)
+    lib.orc_substitute_lanes(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(fv), _pd(bb), _pd(x), int(lanes))
     return x
 
 

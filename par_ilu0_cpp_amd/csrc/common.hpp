@@ -124,13 +124,32 @@ __device__ __forceinline__ void retire_ticket(int* ctr) {
     }
 }
 
-// Sum over G adjacent lanes (G a power of two <= 64); every lane of the
-// group gets the total.  Fixed butterfly order, so results are
-// deterministic for a given G.
+// v of the lane K places up within the lane's row of 16 (DPP row shift; 0
+// past the row's end).
+template <int K>
+__device__ __forceinline__ double dpp_row_shl(double v) {
+    static_assert(K >= 1 && K <= 15, "row shift");
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b & 0xffffffffu), 0x100 | K, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x100 | K, 0xf, 0xf, true);
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                                       static_cast<uint32_t>(lo)));
+}
+
+// Sum over G adjacent lanes (G a power of two <= 64), valid in the group's
+// first lane (the others hold partial sums).  Lane 0 adds lane G/2's value,
+// then G/4's, ..., then lane 1's: the addition tree of the xor butterfly,
+// so the result is deterministic for a given G.  Offsets below 16 are DPP
+// row shifts (no LDS traffic); 16 and 32 cross rows through ds_bpermute.
+// Every lane of the group must be active.
 template <int G>
 __device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    if constexpr (G >= 64) v += __shfl_down(v, 32, 64);
+    if constexpr (G >= 32) v += __shfl_down(v, 16, 64);
+    if constexpr (G >= 16) v += dpp_row_shl<8>(v);
+    if constexpr (G >= 8) v += dpp_row_shl<4>(v);
+    if constexpr (G >= 4) v += dpp_row_shl<2>(v);
+    if constexpr (G >= 2) v += dpp_row_shl<1>(v);
     return v;
 }
 

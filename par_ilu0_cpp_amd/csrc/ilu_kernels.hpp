@@ -427,8 +427,9 @@ struct PassDesc {
 
 template <int G, int S>
 struct PpA {
-    int row, cnt, jj;
-    bool act;
+    int row, cnt, jj, kk;  // row, its entry count, slab column, schedule position
+    int base, count, width;  // the pass's level group
+    bool act, bar;
     int c[S];
 };
 
@@ -443,34 +444,35 @@ template <int G, int S>
 __device__ __forceinline__ void pp_issue_a(const PassDesc& d, int gid, int t, int per,
                                            const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
                                            PpA<G, S>& a) {
-    const int count = d.cb & 0x7fffffff;
+    a.count = d.cb & 0x7fffffff;
+    a.bar = d.cb < 0;
+    a.base = d.base;
     const int j0 = d.jw & 0xffff;
-    const int width = d.jw >> 16;
-    const int nact = count - j0 < per ? count - j0 : per;
+    a.width = d.jw >> 16;
+    const int nact = a.count - j0 < per ? a.count - j0 : per;
     a.act = gid < nact;
-    a.jj = j0 + (a.act ? gid : 0);
-    const int2 ri = rowinfo[d.k_lo + (a.jj - j0)];
+    const int off = a.act ? gid : 0;
+    a.jj = j0 + off;
+    a.kk = d.k_lo + off;
+    const int2 ri = rowinfo[a.kk];
     a.row = ri.x;
     a.cnt = ri.y;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
-        a.c[s] = pcol[q < width ? d.base + q * count + a.jj : 0];
+        a.c[s] = pcol[q < a.width ? a.base + q * a.count + a.jj : 0];
     }
 }
 
 template <int G, int S, typename T, bool UPPER>
-__device__ __forceinline__ void pp_issue_b(const PassDesc& d, int t, int bs, int be, const PpA<G, S>& a,
-                                           const T* __restrict__ pval, const T* __restrict__ pdiag,
-                                           const double* out, PpB<S, T>& b) {
-    const int count = d.cb & 0x7fffffff;
-    const int width = d.jw >> 16;
+__device__ __forceinline__ void pp_issue_b(int t, int bs, int be, const PpA<G, S>& a, const T* __restrict__ pval,
+                                           const T* __restrict__ pdiag, const double* out, PpB<S, T>& b) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
-        b.v[s] = pval[q < width ? d.base + q * count + a.jj : 0];
+        b.v[s] = pval[q < a.width ? a.base + q * a.count + a.jj : 0];
     }
-    if (UPPER) b.dv = pdiag[d.k_lo + (a.jj - (d.jw & 0xffff))];
+    if (UPPER) b.dv = pdiag[a.kk];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
@@ -519,8 +521,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
 #pragma unroll
     for (int u = 0; u < DA; ++u) pp_issue_a<G, S>(sd[u < np ? u : np - 1], gid, t, per, rowinfo, pcol, ra[u]);
 #pragma unroll
-    for (int u = 0; u < DB; ++u)
-        pp_issue_b<G, S, T, UPPER>(sd[u < np ? u : np - 1], t, bs, be, ra[u], pval, pdiag, out, rb[u]);
+    for (int u = 0; u < DB; ++u) pp_issue_b<G, S, T, UPPER>(t, bs, be, ra[u], pval, pdiag, out, rb[u]);
     // np is a multiple of DA: the unrolled body below has no pass guard, so
     // the issue sequence (and the compiler's count of outstanding requests)
     // is the same on every path.
@@ -528,65 +529,62 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
 #pragma unroll
         for (int u = 0; u < DA; ++u) {
             const int q = p + u;
-            {
-                const PassDesc d = sd[q];
-                PpA<G, S>& a = ra[u];
-                PpB<S, T>& r = rb[u % DB];
-                if (a.act) {
-                    double acc = t == 0 ? xs[a.row - bs] : 0.0;
+            // descriptor of the pass issued at the end of this one, read now
+            // so that the LDS latency overlaps the row work
+            const PassDesc dn = sd[q + DA < np ? q + DA : np - 1];
+            PpA<G, S>& a = ra[u];
+            PpB<S, T>& r = rb[u % DB];
+            if (a.act) {
+                double acc = t == 0 ? xs[a.row - bs] : 0.0;
 #pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        if (t + s * G < a.cnt) {
-                            const int c = a.c[s];
-                            double x;
-                            if (c >= bs && c < be) {
-                                x = xs[c - bs];
-                            } else {
-                                uint64_t bits = r.xg[s];
-                                while (bits == kPending) {
-                                    if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
-                                    spin_pause();
-                                    bits = peek_bits(out + c);
-                                }
-                                x = __longlong_as_double(bits);
-                            }
-                            acc = fma(-x, static_cast<double>(r.v[s]), acc);
-                        }
-                    }
-                    const int count = d.cb & 0x7fffffff;
-                    for (int qq = t + S * G; qq < a.cnt; qq += G) {
-                        const int pos = d.base + qq * count + a.jj;
-                        const int c = pcol[pos];
+                for (int s = 0; s < S; ++s) {
+                    if (t + s * G < a.cnt) {
+                        const int c = a.c[s];
                         double x;
                         if (c >= bs && c < be) {
                             x = xs[c - bs];
                         } else {
-                            uint64_t bits;
-                            while ((bits = peek_bits(out + c)) == kPending) {
+                            uint64_t bits = r.xg[s];
+                            while (bits == kPending) {
                                 if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
                                 spin_pause();
+                                bits = peek_bits(out + c);
                             }
                             x = __longlong_as_double(bits);
                         }
-                        acc = fma(-x, static_cast<double>(pval[pos]), acc);
-                    }
-                    const double sum = group_sum<G>(acc);
-                    if (t == 0) {
-                        const double res = UPPER ? sum / static_cast<double>(r.dv) : sum;
-                        xs[a.row - bs] = res;
-                        publish(out + a.row, res);
+                        acc = fma(-x, static_cast<double>(r.v[s]), acc);
                     }
                 }
-                if (d.cb < 0) {
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-                    __builtin_amdgcn_s_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                for (int qq = t + S * G; qq < a.cnt; qq += G) {
+                    const int pos = a.base + qq * a.count + a.jj;
+                    const int c = pcol[pos];
+                    double x;
+                    if (c >= bs && c < be) {
+                        x = xs[c - bs];
+                    } else {
+                        uint64_t bits;
+                        while ((bits = peek_bits(out + c)) == kPending) {
+                            if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
+                            spin_pause();
+                        }
+                        x = __longlong_as_double(bits);
+                    }
+                    acc = fma(-x, static_cast<double>(pval[pos]), acc);
                 }
-                const int qa = q + DA < np ? q + DA : np - 1;
-                pp_issue_a<G, S>(sd[qa], gid, t, per, rowinfo, pcol, ra[u]);
-                const int qb = q + DB < np ? q + DB : np - 1;
-                pp_issue_b<G, S, T, UPPER>(sd[qb], t, bs, be, ra[(u + DB) % DA], pval, pdiag, out, rb[u % DB]);
+                const double sum = group_sum<G>(acc);
+                if (t == 0) {
+                    const double res = UPPER ? sum / static_cast<double>(r.dv) : sum;
+                    xs[a.row - bs] = res;
+                    publish(out + a.row, res);
+                }
             }
+            if (a.bar) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                __builtin_amdgcn_s_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            }
+            pp_issue_a<G, S>(dn, gid, t, per, rowinfo, pcol, ra[u]);
+            pp_issue_b<G, S, T, UPPER>(t, bs, be, ra[(u + DB) % DA], pval, pdiag, out, rb[u % DB]);
         }
     }
     if (timed_out) atomicOr(err, kErrTimeout);

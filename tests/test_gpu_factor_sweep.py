@@ -89,6 +89,29 @@ def test_sweeps_all_lane_shapes(oracle, bits):
                 assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
 
 
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("mode,blk", [(0, 0), (1, 512), (2, 512), (3, 512), (3, 4096)])
+def test_sweeps_bit_exact_in_lane_order(oracle, bits, mode, blk):
+    """Every sweep kernel, at every lanes-per-row shape, reproduces the
+    oracle's lane-order sweep bit for bit (FP32 factors: the oracle gets the
+    same float-rounded values)."""
+    a = gen.convdiff3d(14)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
+    frp, fci, fval = o[0], o[1], o[3]
+    ilu = factored(frp, fci, fval, bits)
+    ofac, dg, _ = oracle.ilu_factorize(frp, fci, fval)
+    if bits == 32:
+        ofac = ofac.astype(np.float32).astype(np.float64)
+    if mode:
+        ilu.SetSweepMode(mode, mode, blk)
+    rng = np.random.default_rng(mode * 10 + bits)
+    for g in (LANES if mode <= 1 else (1, 2, 4, 8, 16)):
+        ilu.SetSweepLanes(g, g)
+        for _ in range(2):
+            b = rng.uniform(-1, 1, a.n)
+            assert np.array_equal(ilu.Substitute(b), oracle.substitute_lanes(frp, fci, dg, ofac, b, g)), g
+
+
 def test_sweep_lane_one_bit_exact(oracle):
     a = gen.poisson2d(50)
     ilu = factored(a.row_ptr, a.col_idx, a.value)

@@ -140,3 +140,56 @@ def test_compensated_factorization_row_sums(oracle, lof):
     if lof == 0:
         # the compensation actually changed something (ILU(0) drops fill here)
         assert not np.array_equal(fac1, plain)
+
+
+def _fma(a, b, c):
+    """Correctly rounded a*b + c (exact rational arithmetic, one rounding)."""
+    from fractions import Fraction
+
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _lane_sweep_reference(rp, ci, dg, fac, b, g):
+    """Pure-Python restatement of the g-lane sweep order: lane t takes
+    entries t, t+g, ... with one rounding per fma, lane 0 starts from the
+    right-hand side, then lane t += lane t+off for off = g/2, ..., 1."""
+    n = rp.size - 1
+    x = [float(v) for v in b]
+
+    def row(init, p0, p1):
+        part = [0.0] * g
+        part[0] = init
+        for p in range(p0, p1):
+            t = (p - p0) % g
+            part[t] = _fma(-x[ci[p]], float(fac[p]), part[t])
+        off = g // 2
+        while off:
+            for t in range(off):
+                part[t] = part[t] + part[t + off]
+            off //= 2
+        return part[0]
+
+    for i in range(n):
+        x[i] = row(x[i], rp[i], dg[i])
+    for i in range(n - 1, -1, -1):
+        x[i] = row(x[i], dg[i] + 1, rp[i + 1]) / float(fac[dg[i]])
+    return np.array(x)
+
+
+@pytest.mark.parametrize("g", [1, 2, 4, 8, 16, 32, 64])
+def test_lane_order_sweep_pinned(oracle, g):
+    """orc_substitute_lanes equals an exact-arithmetic restatement of the
+    lane order for every g, and orc_substitute itself at g = 1."""
+    rp, ci = random_pattern(60, 30, 59, seed=g)
+    va = dominant_values(rp, ci, g)
+    frp, fci, _, fval, _ = oracle.iluk_symbolic(rp, ci, va, 2)
+    fac, dg, _ = oracle.ilu_factorize(frp, fci, fval)
+    b = np.random.default_rng(g).uniform(-1, 1, rp.size - 1)
+    x = oracle.substitute_lanes(frp, fci, dg, fac, b, g)
+    assert np.array_equal(x, _lane_sweep_reference(frp, fci, dg, fac, b, g))
+    if g == 1:
+        assert np.array_equal(x, oracle.substitute(frp, fci, dg, fac, b))
+    else:
+        # a different rounding order: close to, and here not identical with, the serial sweep
+        xs = oracle.substitute(frp, fci, dg, fac, b)
+        assert np.allclose(x, xs, rtol=1e-12, atol=1e-14)
