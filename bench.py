@@ -16,14 +16,19 @@ factorization, GMRES from x0 = 0.  The symbolic pass and the level analysis
 Every step's FP64 true residual is checked.
 
 Forms: "fp64" stores the factors in FP64; "fp32" stores them in FP32 for
-the sweeps (Arnoldi, residuals and the solution stay FP64).  `value` is the
-time per solve of the fastest form that reaches the residual bar.
+the sweeps (Arnoldi, residuals and the solution stay FP64); "fp32-learned"
+adds the learned diagonal compensation: the shipped network predicts one
+relaxation factor per row from A (inside the timed step), the FP32 factors
+are compensated, and plain ILU(1) is solved again if GMRES misses the plain
+FP64 form's step count.  `value` is the time per solve of the fastest form
+that reaches the residual bar.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -47,12 +52,13 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--side", type=int, default=128, help="grid side (128 = the benchmark workload)")
-    p.add_argument("--forms", default="fp64,fp32")
+    p.add_argument("--forms", default="fp64,fp32,fp32-learned")
     p.add_argument("--cpu-steps", type=int, default=12,
                    help="Arnoldi steps in the bounded CPU baseline sample (0 = skip)")
     p.add_argument("--no-autotune", action="store_true", help="keep the default sweep shape")
     p.add_argument("--sweep", default="",
-                   help="fixed sweep shape MODE_L:LANES_L:MODE_U:LANES_U:BLOCK_ROWS instead of autotuning")
+                   help="fixed sweep shape MODE_L:LANES_L:MODE_U:LANES_U:BLOCK instead of autotuning, "
+                        "BLOCK = rows of contiguous blocks or a tile TXxTYxTZ")
     p.add_argument("--traffic", default=str(ROOT / "profiles" / "sweep_traffic_convdiff128.txt"),
                    help="rocprofv3 summary holding the measured HBM bytes per apply")
     p.add_argument("--copy-calib", action="store_true",
@@ -60,7 +66,7 @@ def parse():
     return p.parse_args()
 
 
-SWEEP_KERNELS = {0: "k_sweep", 1: "k_sweep_blk", 2: "k_sweep_pk", 3: "k_sweep_pp"}
+SWEEP_KERNELS = {0: "k_sweep", 1: "k_sweep_blk", 2: "k_sweep_pp"}
 
 
 def copy_calibration():
@@ -123,7 +129,17 @@ def main():
 
     results = {}
     for form in [f.strip() for f in args.forms.split(",") if f.strip()]:
-        bits = 32 if form == "fp32" else 64
+        bits = 32 if form.startswith("fp32") else 64
+        learned = form.endswith("learned")
+        net = None
+        if learned:
+            from par_ilu0_cpp_amd.learned import WEIGHTS, Gnn
+            if not WEIGHTS.exists():
+                log(f"[bench] {form}: no trained weights at {WEIGHTS}; skipped")
+                continue
+            net = Gnn.load(WEIGHTS)
+            budget = results["fp64"]["its"][0] if "fp64" in results else 2000
+            d_a = P._dev_csr(a.n, d_rp.data_ptr(), d_ci.data_ptr(), d_va.data_ptr())
         ilu = P.IluSolver(0)
         ilu.SetFactorPrecision(bits)
         torch.cuda.synchronize()
@@ -136,11 +152,14 @@ def main():
         nl, nu = ilu.Levels()
         log(f"[bench] {form}: symbolic+analysis {t_setup*1e3:.1f} ms nnz(F)={nnz_f} levels L={nl} U={nu}")
         sweep_cfg = ilu.SweepConfig()
+        ilu.SetGridShape(s, s, s)  # the workload's grid: lets the sweeps use 3D tiles
         if args.sweep:
-            ml, ll, mu, lu, blk = (int(v) for v in args.sweep.split(":"))
-            ilu.SetSweepLanes(ll, lu)
-            if ml or mu:
-                ilu.SetSweepMode(ml, mu, blk)
+            ml, ll, mu, lu, blk = args.sweep.split(":")
+            ilu.SetSweepLanes(int(ll), int(lu))
+            if "x" in blk:
+                ilu.SetSweepTiles(int(ml), int(mu), *(int(v) for v in blk.split("x")))
+            elif int(ml) or int(mu):
+                ilu.SetSweepMode(int(ml), int(mu), int(blk))
             sweep_cfg = ilu.SweepConfig()
             log(f"[bench] {form}: fixed sweep shape {sweep_cfg}")
         elif not args.no_autotune:
@@ -158,8 +177,20 @@ def main():
         prm.krylov_subspace_dimension = 50
         prm.max_iterations = 2000
 
+        fallbacks = []
+
         def step():
             d_x.zero_()
+            if learned:
+                # network -> omega, compensated factorization, GMRES; plain
+                # ILU(1) again past the step budget (GmresSolveLearnedDeviceEx)
+                it, fb = ctypes.c_int(0), ctypes.c_int(0)
+                ok = P.fn("GmresSolveLearnedDeviceEx")(g.hdl, ilu.hdl, net.hdl, ctypes.byref(d_a),
+                                                       ctypes.c_void_p(d_b.data_ptr()),
+                                                       ctypes.c_void_p(d_x.data_ptr()), int(budget),
+                                                       ctypes.byref(it), ctypes.byref(fb))
+                fallbacks.append(bool(fb.value))
+                return bool(ok), it.value
             if ilu.FactorizeDevice(d_va.data_ptr(), True) != 0:
                 raise SystemExit("factorization failed")
             return g.SolveDevice(a.n, d_rp.data_ptr(), d_ci.data_ptr(), d_va.data_ptr(), d_b.data_ptr(),
@@ -187,12 +218,13 @@ def main():
         bt = tri_bytes(a.n, nnz_f, bits // 8)
         results[form] = dict(
             s_per_solve=per, its=its_all, true_rel_res=max(res_all), ok=all(oks), setup_s=t_setup,
+            fallbacks=fallbacks[-args.steps:] if learned else None,
             nnz_f=nnz_f, levels=[nl, nu], apply_s=t_apply, mv_s=t_mv, tri_bytes=bt, sweep=sweep_cfg,
             tri_gbs=bt / t_apply / 1e9 if t_apply > 0 else None,
         )
         log(f"[bench] {form}: {per*1e3:.1f} ms/solve its={its_all} rel_res={max(res_all):.2e} "
             f"apply={t_apply*1e6:.1f} us ({results[form]['tri_gbs']:.1f} GB/s) mv={t_mv*1e6:.1f} us")
-        del g, ilu
+        del g, ilu, net
 
     valid = {f: r for f, r in results.items() if r["ok"] and r["true_rel_res"] <= 1e-8}
     best = min(valid, key=lambda f: valid[f]["s_per_solve"]) if valid else None
@@ -224,9 +256,10 @@ def main():
             "setup_s": r["setup_s"], "parallelism": "single GPU", "sweep_shape": r["sweep"],
         },
         "forms": {f: {"s_per_solve": v["s_per_solve"], "its": v["its"], "true_rel_res": v["true_rel_res"],
-                      "apply_us": v["apply_s"] * 1e6, "trisolve_gbs": v["tri_gbs"]} for f, v in results.items()},
-        "speedup_fp32_vs_fp64": (results["fp64"]["s_per_solve"] / results["fp32"]["s_per_solve"]
-                                 if "fp64" in results and "fp32" in results else None),
+                      "apply_us": v["apply_s"] * 1e6, "trisolve_gbs": v["tri_gbs"],
+                      "speedup_vs_fp64": (results["fp64"]["s_per_solve"] / v["s_per_solve"]
+                                          if "fp64" in results else None),
+                      "fallbacks": v["fallbacks"]} for f, v in results.items()},
         "roofline": {
             "kernel": "triangular sweep apply: {} (L) + {} (U)".format(
                 SWEEP_KERNELS.get(r["sweep"]["mode_l"], "?"), SWEEP_KERNELS.get(r["sweep"]["mode_u"], "?")),

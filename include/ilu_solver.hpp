@@ -66,19 +66,28 @@ public:
     // adds omega[i] times the sum of its dropped fill updates to its pivot.
     void SetDiagonalCompensation(const float* d_omega);
     // Times the sweeps in every launch shape -- level-scheduled rows with
-    // 1..64 lanes per row, and block-local sweeps (blocks of 4096, 8192 or
-    // 16384 rows staged in LDS; CSR or packed factors) -- over `reps` applies
-    // each, and keeps the fastest shape for each direction.  Results depend
-    // on the lanes per row only through the rounding of row sums.
-    // table (optional, cap rows of 5 doubles): mode, block rows, lanes,
-    // forward ms, backward ms.  Returns the number of shapes timed.
+    // 1..64 lanes per row, and block-local sweeps over contiguous blocks of
+    // 4096, 8192 or 16384 rows (CSR or packed factors) and, once the grid
+    // shape is declared, over 3D tiles (packed factors) -- over `reps`
+    // applies each, and keeps the fastest shape for each direction.
+    // Results depend on the lanes per row only through the rounding of row
+    // sums.  table (optional, cap rows of 8 doubles): mode, block rows, tile
+    // x, y, z, lanes, forward ms, backward ms.  Returns the shapes timed.
     int AutotuneSweeps(int reps, double* table, int cap);
     // Sweep algorithm per direction: 0 level-scheduled rows, 1 block-local
-    // with blocks of blk_rows rows (64..16384), 2 block-local over a packed,
-    // schedule-ordered copy of the factors (refreshed at every Factorize),
-    // 3 the packed sweep with its loads issued several passes ahead.
+    // over contiguous blocks of blk_rows rows (64..16384), 2 block-local over
+    // a packed, schedule-ordered copy of the factors (refreshed at every
+    // Factorize) with its loads issued several passes ahead.
     void SetSweepMode(int mode_l, int mode_u, int blk_rows);
-    // cfg[5]: mode_l, lanes_l, mode_u, lanes_u, blk_rows.
+    // Declares that row x + nx (y + ny z) is grid cell (x, y, z); enables
+    // tiled partitions of the block-local sweeps.
+    void SetGridShape(int nx, int ny, int nz);
+    // Block-local sweeps (mode 0 or 2 per direction) over tx x ty x tz
+    // tiles of the declared grid (at most 16384 rows per tile).  Blocks run
+    // in a topological order of their dependency graph; the results are
+    // those of the same lanes per row with any other partition.
+    void SetSweepTiles(int mode_l, int mode_u, int tx, int ty, int tz);
+    // cfg[8]: mode_l, lanes_l, mode_u, lanes_u, blk_rows, tile x, y, z.
     void GetSweepConfig(int* cfg) const;
     // Device pointers of the current pattern and factors (valid after
     // SetupMatrix / Factorize): row_ptr, col_idx, diag_ptr, factor values.

@@ -34,19 +34,29 @@ int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
 int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
 
 /* Autotune the sweeps after Factorize: level-scheduled rows with 1..64
- * lanes per row and the block-local sweeps (4096/8192/16384-row blocks in
- * LDS; CSR, packed and pipelined packed factors) with 2..32 lanes are each
- * timed over `reps` applies; the fastest shape per direction is kept.  table (optional, cap rows x 5 doubles): mode,
- * block rows, lanes, forward ms, backward ms.  Returns the rows timed. */
+ * lanes per row and the block-local sweeps (contiguous blocks of
+ * 4096/8192/16384 rows, and 3D tiles once the grid shape is declared; CSR or
+ * packed factors) with 2..32 lanes are each timed over `reps` applies; the
+ * fastest shape per direction is kept.  table (optional, cap rows x 8
+ * doubles): mode, block rows, tile x, tile y, tile z, lanes, forward ms,
+ * backward ms.  Returns the rows timed. */
 #define IluSolverAutotuneEx LLJBASH_DECL(IluSolverAutotuneEx)
 int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap);
 
-/* Explicit sweep algorithm per direction (0 level rows, 1 block-local with
- * blocks of blk_rows rows, 2 block-local over packed factors, 3 the same
- * software-pipelined) and the current configuration
- * (cfg[5]: mode_l, lanes_l, mode_u, lanes_u, blk_rows). */
+/* Explicit sweep algorithm per direction (0 level rows, 1 block-local over
+ * contiguous blocks of blk_rows rows, 2 block-local over packed factors,
+ * software-pipelined) and the current configuration (cfg[8]: mode_l,
+ * lanes_l, mode_u, lanes_u, blk_rows, tile x, tile y, tile z). */
 #define IluSolverSetSweepModeEx LLJBASH_DECL(IluSolverSetSweepModeEx)
 int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_rows);
+/* Declares the natural-order grid of the matrix (row x + nx (y + ny z));
+ * nx * ny * nz must equal n.  Enables tiled block-local sweeps. */
+#define IluSolverSetGridShapeEx LLJBASH_DECL(IluSolverSetGridShapeEx)
+int IluSolverSetGridShapeEx(IluSolverHdl hdl, int nx, int ny, int nz);
+/* Block-local sweeps (mode 0 or 2 per direction) over tx x ty x tz tiles of
+ * the declared grid (at most 16384 rows per tile). */
+#define IluSolverSetSweepTilesEx LLJBASH_DECL(IluSolverSetSweepTilesEx)
+int IluSolverSetSweepTilesEx(IluSolverHdl hdl, int mode_l, int mode_u, int tx, int ty, int tz);
 #define IluSolverGetSweepConfigEx LLJBASH_DECL(IluSolverGetSweepConfigEx)
 int IluSolverGetSweepConfigEx(IluSolverHdl hdl, int* cfg);
 

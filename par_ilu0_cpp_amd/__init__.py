@@ -183,16 +183,25 @@ class IluSolver:
 
     def Autotune(self, reps: int = 3):
         """Times every sweep shape and keeps the fastest per direction.
-        -> (config dict, table[rows, 5]: mode, block rows, lanes, fwd ms, bwd ms)."""
-        table = np.zeros((64, 5), dtype=np.float64)
-        rc = fn("IluSolverAutotuneEx")(self.hdl, int(reps), _ptr(table, C.c_double), 64)
+        -> (config dict, table[rows, 8]: mode, block rows, tile x, y, z,
+        lanes, fwd ms, bwd ms)."""
+        table = np.zeros((96, 8), dtype=np.float64)
+        rc = fn("IluSolverAutotuneEx")(self.hdl, int(reps), _ptr(table, C.c_double), 96)
         if rc < 0:
             raise RuntimeError("IluSolverAutotuneEx failed")
-        return self.SweepConfig(), table[:rc].copy()
+        return self.SweepConfig(), table[: min(rc, 96)].copy()
 
     def SetSweepMode(self, mode_l: int, mode_u: int, blk_rows: int = 0) -> None:
         if fn("IluSolverSetSweepModeEx")(self.hdl, int(mode_l), int(mode_u), int(blk_rows)) != 0:
             raise RuntimeError("SetSweepMode failed")
+
+    def SetGridShape(self, nx: int, ny: int, nz: int) -> None:
+        if fn("IluSolverSetGridShapeEx")(self.hdl, int(nx), int(ny), int(nz)) != 0:
+            raise RuntimeError("SetGridShape failed")
+
+    def SetSweepTiles(self, mode_l: int, mode_u: int, tx: int, ty: int, tz: int) -> None:
+        if fn("IluSolverSetSweepTilesEx")(self.hdl, int(mode_l), int(mode_u), int(tx), int(ty), int(tz)) != 0:
+            raise RuntimeError("SetSweepTiles failed")
 
     def Schedule(self) -> dict:
         """Schedules and sweep configuration (IluSolver::DescribeSchedule)."""
@@ -204,10 +213,10 @@ class IluSolver:
         return json.loads(buf.value.decode())
 
     def SweepConfig(self) -> dict:
-        cfg = np.zeros(5, dtype=np.int32)
+        cfg = np.zeros(8, dtype=np.int32)
         fn("IluSolverGetSweepConfigEx")(self.hdl, _ptr(cfg, C.c_int))
         return dict(mode_l=int(cfg[0]), lanes_l=int(cfg[1]), mode_u=int(cfg[2]), lanes_u=int(cfg[3]),
-                    blk_rows=int(cfg[4]))
+                    blk_rows=int(cfg[4]), tile=[int(cfg[5]), int(cfg[6]), int(cfg[7])])
 
     def Levels(self) -> tuple[int, int]:
         a, b = C.c_int(0), C.c_int(0)

@@ -88,6 +88,8 @@ SIGNATURES = [
     ("IluSolverScheduleJsonEx", _int, [_vp, C.c_char_p, _int]),
     ("IluSolverAutotuneEx", _int, [_vp, _int, P(_dbl), _int]),
     ("IluSolverSetSweepModeEx", _int, [_vp, _int, _int, _int]),
+    ("IluSolverSetGridShapeEx", _int, [_vp, _int, _int, _int]),
+    ("IluSolverSetSweepTilesEx", _int, [_vp, _int, _int, _int, _int, _int]),
     ("IluSolverGetSweepConfigEx", _int, [_vp, P(_int)]),
     ("IluSolverSetDiagonalCompensationEx", _int, [_vp, _vp]),
     ("IluSolverSymbolicDeviceEx", _int, [_vp, P(CsrMatrix), _int]),

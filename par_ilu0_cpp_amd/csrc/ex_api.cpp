@@ -52,6 +52,14 @@ int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_ro
     return guarded("SetSweepMode", [&] { S(hdl)->SetSweepMode(mode_l, mode_u, blk_rows); return 0; });
 }
 
+int IluSolverSetGridShapeEx(IluSolverHdl hdl, int nx, int ny, int nz) {
+    return guarded("SetGridShape", [&] { S(hdl)->SetGridShape(nx, ny, nz); return 0; });
+}
+
+int IluSolverSetSweepTilesEx(IluSolverHdl hdl, int mode_l, int mode_u, int tx, int ty, int tz) {
+    return guarded("SetSweepTiles", [&] { S(hdl)->SetSweepTiles(mode_l, mode_u, tx, ty, tz); return 0; });
+}
+
 int IluSolverGetSweepConfigEx(IluSolverHdl hdl, int* cfg) {
     S(hdl)->GetSweepConfig(cfg);
     return 0;

@@ -30,20 +30,22 @@ int lanes_for_avg(double avg_len);
 }  // namespace pilu
 
 namespace pilu {
-struct PackPass;
+// Largest block of the block-local sweeps (its x lives in LDS).
+constexpr int kMaxBlockRows = 16384;
+
 // Schedule-ordered copy of one direction's factors for the packed sweep.
 struct SweepPack {
     DevBuf<int> rowinfo;            // (row, entry count) per schedule position
-    DevBuf<int> pcol, psrc;         // slab columns and source CSR positions
+    DevBuf<int> pcol, psrc;         // slab columns (in-block: -(1 + block slot)) and source CSR positions
+    DevBuf<int> kptr;               // first schedule position of every processed block
     DevBuf<double> pval64, pdiag64;
     DevBuf<float> pval32, pdiag32;
     std::vector<int> h_blev, h_loff, h_base, h_width;  // host copies for pass lists
     long ne = 0;
     int max_w = 0;
-    int blk = 0;                    // block rows the pack was built for
-    DevBuf<int> pass_ptr;
-    DevBuf<char> passes;            // PackPass records
-    DevBuf<char> pdesc;             // PassDesc records (same passes, compact, padded per block)
+    int version = -1;               // partition the pack was built for
+    long factors = -1;              // factorization its values were gathered from
+    DevBuf<char> pdesc;             // PassDesc records, each block's list padded
     DevBuf<int> desc_ptr;           // first PassDesc of each block
     int max_np = 0;                 // most PassDesc records of one block
     int pass_g = 0;                 // lanes per row the pass list was built for
@@ -76,11 +78,18 @@ struct lljbash::IluSolver::Ext {
     int precision = 64;
     int lanes_l = 0, lanes_u = 0;  // 0 = heuristic
     // Sweep algorithm per direction: 0 = level-scheduled rows (k_sweep),
-    // 1 = block-local (k_sweep_blk) with blocks of blk_rows rows,
-    // 2 = packed block-local (k_sweep_pk), 3 = pipelined packed (k_sweep_pp).
+    // 1 = block-local over contiguous blocks of blk_rows rows (k_sweep_blk),
+    // 2 = packed block-local over the current partition (k_sweep_pp).
     int mode_l = 0, mode_u = 0;
     pilu::SweepPack pack_l, pack_u;
-    int blk_rows = 0, nblk = 0;
+    // Block partition of the rows: contiguous blocks of blk_rows rows, or
+    // tiles of the grid (tile[] != 0, grid[] declared by SetGridShape).
+    int blk_rows = 0, nblk = 0, max_block_rows = 0;
+    int tile[3] = {0, 0, 0}, grid[3] = {0, 0, 0};
+    std::vector<int> h_block_of;
+    pilu::DevBuf<int> block_of;
+    int part_version = 0;
+    long factor_version = 0;
     pilu::DevBuf<int> bperm_l, bperm_u, blev_l, blev_u, loff_l, loff_u;
     bool factored = false;
     bool y_armed = false;

@@ -86,12 +86,13 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
                                                    const int* __restrict__ ci,
                                                    const int* __restrict__ dg, int* level,
                                                    int* maxlev, int* ctr, int* err, uint64_t wd_limit,
-                                                   int blk) {
+                                                   const int* __restrict__ block_of) {
     const int ticket = take_ticket(ctr);
     const Watchdog wd(wd_limit);
     const int k = ticket * kBlock + threadIdx.x;
     const bool active = k < n;
     const int row = active ? (UPPER ? n - 1 - k : k) : 0;
+    const int my_block = block_of && active ? block_of[row] : 0;
     int e = 0, end = 0;
     if (active) {
         e = UPPER ? dg[row] + 1 : rp[row];
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
         if (!done) {
             while (e < end) {
                 const int c = ci[e];
-                if (blk > 0 && c / blk != row / blk) { ++e; continue; }  // outside the row's block
+                if (block_of && block_of[c] != my_block) { ++e; continue; }  // outside the row's block
                 const int d = peek_int(level + c);
                 if (d < 0) break;
                 lv = d + 1 > lv ? d + 1 : lv;
@@ -274,146 +275,35 @@ __global__ __launch_bounds__(512) void k_sweep_blk(int n, int blk, int nblk, con
 }
 
 // ---------------------------------------------------------------------
-// Packed block-local sweep.  Same schedule as k_sweep_blk (blocks of blk
-// rows, local levels, LDS-resident block results), but the factors are
-// also stored in schedule order: for every (block, local level) group the
-// entries of its rows form a column-major ELL slab (entry q of the group's
-// j-th row at base + q * count + j), so a level needs one round of loads
-// that depends on nothing but the pass record.  Each workgroup walks its
-// block as a list of passes (up to 512/G rows of one level group) and loads
-// pass p+1 while computing pass p; workgroup barriers, LDS-only, sit at
-// level boundaries.  Row sums follow k_sweep's lane order: bit-identical
-// results for the same G.
-// ---------------------------------------------------------------------
-struct PackPass {
-    int k_lo, k_hi;  // schedule positions of the pass
-    int gstart;      // first position of the pass's level group
-    int base;        // slab offset of the group
-    int count;       // rows in the group (slab column stride)
-    int width;       // entries of the group's longest row
-    int bar;         // 1: a level boundary follows this pass
-    int pad;
-};
-
-template <int S, typename T>
-struct PackRow {
-    int row = 0, cnt = 0, j = 0;
-    bool act = false;
-    double bv = 0.0;
-    T dv = T(1);
-    int c[S];
-    T v[S];
-};
-
-template <int G, int S, typename T, bool UPPER>
-__device__ __forceinline__ void pk_load(const PackPass& pr, int gid, int t, const int2* __restrict__ rowinfo,
-                                        const int* __restrict__ pcol, const T* __restrict__ pval,
-                                        const T* __restrict__ pdiag, const double* b, PackRow<S, T>& r) {
-    const int k = pr.k_lo + gid;
-    r.act = k < pr.k_hi;
-    if (!r.act) return;
-    r.j = k - pr.gstart;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int q = t + s * G;
-        if (q < pr.width) {
-            const int pos = pr.base + q * pr.count + r.j;
-            r.c[s] = pcol[pos];
-            r.v[s] = pval[pos];
-        }
-    }
-    const int2 ri = rowinfo[k];
-    r.row = ri.x;
-    r.cnt = ri.y;
-    if (UPPER) r.dv = pdiag[k];
-    if (t == 0) r.bv = b[r.row];
-}
-
-template <int G, int S, typename T, bool UPPER>
-__global__ __launch_bounds__(512) void k_sweep_pk(int n, int blk, int nblk, const int* __restrict__ pass_ptr,
-                                                  const PackPass* __restrict__ passes,
-                                                  const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
-                                                  const T* __restrict__ pval, const T* __restrict__ pdiag,
-                                                  const double* b, double* out, double* reset, const int* gate,
-                                                  int* ctr, int* err, uint64_t wd_limit) {
-    extern __shared__ double xs[];
-    if (gate && *gate == 0) return;
-    const int ticket = take_ticket(ctr);
-    const Watchdog wd(wd_limit);
-    const int bid = UPPER ? nblk - 1 - ticket : ticket;
-    const int bs = bid * blk;
-    const int be = bs + blk < n ? bs + blk : n;
-    const int gid = threadIdx.x / G;
-    const int t = threadIdx.x & (G - 1);
-    const int p0 = pass_ptr[ticket], p1 = pass_ptr[ticket + 1];
-    bool timed_out = false;
-    PackRow<S, T> cur, nxt;
-    PackPass pc = passes[p0];
-    pk_load<G, S, T, UPPER>(pc, gid, t, rowinfo, pcol, pval, pdiag, b, cur);
-    for (int p = p0; p < p1; ++p) {
-        PackPass pn = pc;
-        if (p + 1 < p1) {
-            pn = passes[p + 1];
-            pk_load<G, S, T, UPPER>(pn, gid, t, rowinfo, pcol, pval, pdiag, b, nxt);
-        }
-        if (cur.act) {
-            double acc = t == 0 ? cur.bv : 0.0;
-            if (t == 0 && reset) reset[cur.row] = __longlong_as_double(kPending);
-            auto fetch_x = [&](int c) -> double {
-                if (c >= bs && c < be) return xs[c - bs];
-                uint64_t bits;
-                while ((bits = peek_bits(out + c)) == kPending) {
-                    if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
-                    spin_pause();
-                }
-                return __longlong_as_double(bits);
-            };
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (t + s * G < cur.cnt) acc = fma(-fetch_x(cur.c[s]), static_cast<double>(cur.v[s]), acc);
-            }
-            for (int q = t + S * G; q < cur.cnt; q += G) {
-                const int pos = pc.base + q * pc.count + cur.j;
-                acc = fma(-fetch_x(pcol[pos]), static_cast<double>(pval[pos]), acc);
-            }
-            const double sum = group_sum<G>(acc);
-            if (t == 0) {
-                const double r = UPPER ? sum / static_cast<double>(cur.dv) : sum;
-                xs[cur.row - bs] = r;
-                publish(out + cur.row, r);
-            }
-        }
-        if (pc.bar) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_s_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        }
-        pc = pn;
-        cur = nxt;
-    }
-    if (timed_out) atomicOr(err, kErrTimeout);
-    retire_ticket(ctr);
-}
-
-// ---------------------------------------------------------------------
-// Pipelined packed sweep.  Same blocks, pack and summation order as
-// k_sweep_pk, reorganised so that no pass waits on memory it could have
-// requested earlier:
+// Packed block-local sweep.  The rows are split into blocks (contiguous row
+// ranges, or any partition whose block dependency graph is acyclic, such
+// as 3D tiles of a grid); each workgroup takes one block in a topological
+// order of that graph and solves it level by level, the block's results
+// held in LDS.  The factors are stored in the sweep's schedule order: for
+// every (block, local level) group the entries of its rows form a
+// column-major ELL slab (entry q of the group's j-th row at
+// base + q * count + j).  A column inside the row's own block is stored as
+// -(1 + its schedule position within the block), its LDS slot; any other
+// column as itself, read from the output vector as a value-flag.
+//
+// Everything a pass needs is requested before it runs:
 //   * at block start the right-hand side of the block's rows is staged in
 //     LDS (xs[] then holds b_i until row i is solved and x_i afterwards;
-//     this also re-arms the companion buffer for the block's rows in bulk),
+//     the companion buffer is re-armed for those rows at the same time),
 //     and so are the block's compact pass descriptors (PassDesc);
 //   * the loads of pass t+DA (row info, column indices) and of pass t+DB
-//     (values, 1/diagonal, and a first read of every x_c outside the block)
-//     are issued right after pass t computes, in the order they will be
+//     (values, diagonal, and a first read of every out-of-block x_c) are
+//     issued right after pass t computes, in the order they will be
 //     consumed, so the in-order vector-memory counter never makes a pass
-//     wait for a younger request; all addresses are clamped into the pack
-//     so the issue sequence is branch-free;
-//   * an out-of-block x_c read early that was still pending is polled again
-//     when its pass runs.
-// Every entry is accumulated in the same order as in k_sweep / k_sweep_pk
-// with G lanes per row, so the results are bit-identical to theirs.
+//     wait for a younger request; addresses are clamped into the pack so
+//     that the issue sequence has no branches;
+//   * an out-of-block x_c that was still pending when read early is polled
+//     again when its pass runs.
+// Workgroup barriers, LDS-only, sit at level boundaries.  Every row sum
+// follows k_sweep's lane order, so results are bit-identical to it for
+// the same lanes per row G.
 // ---------------------------------------------------------------------
+
 // Passes per unrolled step of k_sweep_pp; every block's descriptor list is
 // padded to a multiple of it with empty passes (cb = 0: no rows).
 constexpr int kPpUnroll = 8;
@@ -465,7 +355,7 @@ __device__ __forceinline__ void pp_issue_a(const PassDesc& d, int gid, int t, in
 }
 
 template <int G, int S, typename T, bool UPPER>
-__device__ __forceinline__ void pp_issue_b(int t, int bs, int be, const PpA<G, S>& a, const T* __restrict__ pval,
+__device__ __forceinline__ void pp_issue_b(int t, int idle, const PpA<G, S>& a, const T* __restrict__ pval,
                                            const T* __restrict__ pdiag, const double* out, PpB<S, T>& b) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -477,34 +367,36 @@ __device__ __forceinline__ void pp_issue_b(int t, int bs, int be, const PpA<G, S
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
         const int c = a.c[s];
-        // lanes with nothing to read early all read x[bs] (one request)
-        const bool need = a.act && q < a.cnt && (c < bs || c >= be);
-        int idx = need ? c : bs;
+        // lanes with nothing to read early all read one slot of the block
+        // (a single request per wave)
+        const bool need = a.act && q < a.cnt && c >= 0;
+        int idx = need ? c : idle;
         asm volatile("" : "+v"(idx));  // keep the read unconditional (a select, not a branch)
         b.xg[s] = peek_bits(out + idx);
     }
 }
 
+// kptr[t]: schedule position of the first row of the t-th block processed;
+// pass_ptr[t]: its first PassDesc (each list padded to a multiple of DA).
 template <int G, int S, int DA, int DB, typename T, bool UPPER>
-__global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, const int* __restrict__ pass_ptr,
+__global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, const int* __restrict__ pass_ptr,
                                                   const PassDesc* __restrict__ desc,
                                                   const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
                                                   const T* __restrict__ pval, const T* __restrict__ pdiag,
                                                   const double* b, double* out, double* reset, const int* gate,
-                                                  int* ctr, int* err, uint64_t wd_limit) {
+                                                  int* ctr, int* err, uint64_t wd_limit, int max_rows) {
     static_assert(DA % DB == 0 && DA > DB && DA == kPpUnroll, "ring sizes");
     extern __shared__ double xs[];
     if (gate && *gate == 0) return;
     const int ticket = take_ticket(ctr);
     const Watchdog wd(wd_limit);
-    const int bid = UPPER ? nblk - 1 - ticket : ticket;
-    const int bs = bid * blk;
-    const int be = bs + blk < n ? bs + blk : n;
-    PassDesc* sd = reinterpret_cast<PassDesc*>(xs + blk);
+    const int kb0 = kptr[ticket], nrows = kptr[ticket + 1] - kb0;
+    PassDesc* sd = reinterpret_cast<PassDesc*>(xs + max_rows);
     const int p0 = pass_ptr[ticket], np = pass_ptr[ticket + 1] - p0;
-    for (int i = threadIdx.x; i < be - bs; i += 512) {
-        xs[i] = b[bs + i];
-        if (reset) reset[bs + i] = __longlong_as_double(kPending);
+    for (int i = threadIdx.x; i < nrows; i += 512) {
+        const int row = rowinfo[kb0 + i].x;
+        xs[i] = b[row];
+        if (reset) reset[row] = __longlong_as_double(kPending);
     }
     for (int i = threadIdx.x; i < np; i += 512) sd[i] = desc[p0 + i];
     __syncthreads();
@@ -512,6 +404,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
         retire_ticket(ctr);
         return;
     }
+    const int idle = rowinfo[kb0].x;
     constexpr int per = 512 / G;
     const int gid = threadIdx.x / G;
     const int t = threadIdx.x & (G - 1);
@@ -521,7 +414,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
 #pragma unroll
     for (int u = 0; u < DA; ++u) pp_issue_a<G, S>(sd[u < np ? u : np - 1], gid, t, per, rowinfo, pcol, ra[u]);
 #pragma unroll
-    for (int u = 0; u < DB; ++u) pp_issue_b<G, S, T, UPPER>(t, bs, be, ra[u], pval, pdiag, out, rb[u]);
+    for (int u = 0; u < DB; ++u) pp_issue_b<G, S, T, UPPER>(t, idle, ra[u], pval, pdiag, out, rb[u]);
     // np is a multiple of DA: the unrolled body below has no pass guard, so
     // the issue sequence (and the compiler's count of outstanding requests)
     // is the same on every path.
@@ -535,14 +428,15 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
             PpA<G, S>& a = ra[u];
             PpB<S, T>& r = rb[u % DB];
             if (a.act) {
-                double acc = t == 0 ? xs[a.row - bs] : 0.0;
+                const int slot = a.kk - kb0;
+                double acc = t == 0 ? xs[slot] : 0.0;
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     if (t + s * G < a.cnt) {
                         const int c = a.c[s];
                         double x;
-                        if (c >= bs && c < be) {
-                            x = xs[c - bs];
+                        if (c < 0) {
+                            x = xs[-1 - c];
                         } else {
                             uint64_t bits = r.xg[s];
                             while (bits == kPending) {
@@ -559,8 +453,8 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
                     const int pos = a.base + qq * a.count + a.jj;
                     const int c = pcol[pos];
                     double x;
-                    if (c >= bs && c < be) {
-                        x = xs[c - bs];
+                    if (c < 0) {
+                        x = xs[-1 - c];
                     } else {
                         uint64_t bits;
                         while ((bits = peek_bits(out + c)) == kPending) {
@@ -574,7 +468,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
                 const double sum = group_sum<G>(acc);
                 if (t == 0) {
                     const double res = UPPER ? sum / static_cast<double>(r.dv) : sum;
-                    xs[a.row - bs] = res;
+                    xs[slot] = res;
                     publish(out + a.row, res);
                 }
             }
@@ -584,7 +478,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(int n, int blk, int nblk, cons
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             }
             pp_issue_a<G, S>(dn, gid, t, per, rowinfo, pcol, ra[u]);
-            pp_issue_b<G, S, T, UPPER>(t, bs, be, ra[(u + DB) % DA], pval, pdiag, out, rb[u % DB]);
+            pp_issue_b<G, S, T, UPPER>(t, idle, ra[(u + DB) % DA], pval, pdiag, out, rb[u % DB]);
         }
     }
     if (timed_out) atomicOr(err, kErrTimeout);

@@ -86,10 +86,10 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
     const int grid = grid_for(n, kBlock);
     if (upper)
         k_levels<true><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
-                                               e->err.get() + kErrLevels, wd, 0);
+                                               e->err.get() + kErrLevels, wd, nullptr);
     else
         k_levels<false><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
-                                                e->err.get() + kErrLevels, wd, 0);
+                                                e->err.get() + kErrLevels, wd, nullptr);
     PILU_HIP(hipGetLastError());
     const int nlev = read_int(maxlev, s) + 1;
     if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
@@ -110,34 +110,34 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
     return nlev;
 }
 
-__global__ void k_block_keys(int n, int blk, int nblk, const int* __restrict__ level, bool upper, int lbits,
-                             unsigned* __restrict__ keys) {
+__global__ void k_block_keys(int n, const int* __restrict__ block_of, const int* __restrict__ pos,
+                             const int* __restrict__ level, int lbits, unsigned* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const unsigned b = static_cast<unsigned>(i / blk);
-    const unsigned p = upper ? static_cast<unsigned>(nblk) - 1u - b : b;
-    keys[i] = (p << lbits) | static_cast<unsigned>(level[i]);
+    keys[i] = (static_cast<unsigned>(pos[block_of[i]]) << lbits) | static_cast<unsigned>(level[i]);
 }
 
-// Schedules of the block-local sweep: local levels inside blocks of `blk`
-// rows, rows sorted by (processing block, local level, row), and the group
-// offsets of every (block, level).
-void build_block_schedule(Ext* e, bool upper, int blk, DevBuf<int>& perm, DevBuf<int>& blev, DevBuf<int>& loff) {
+// Schedule of the block-local sweeps in one direction: levels restricted to
+// each block's own rows, rows sorted by (processing position of their
+// block, local level, row), and the group offsets of every (block, level).
+// pos[b]: processing position of block b.
+void build_block_schedule(Ext* e, bool upper, const std::vector<int>& pos, DevBuf<int>& perm, DevBuf<int>& blev,
+                          DevBuf<int>& loff) {
     const int n = e->n;
+    const int nblk = e->nblk;
     hipStream_t s = e->stream;
-    const int nblk = ceil_div(n, blk);
     int* level = e->level.get();
     int* maxlev = e->err.get() + kErrMaxLev;
     k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
     PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
     const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
+    const int* bo = e->block_of.get();
     if (upper)
         k_levels<true><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev,
-                                                              e->ctr.get(), e->err.get() + kErrLevels, wd, blk);
+                                                              e->ctr.get(), e->err.get() + kErrLevels, wd, bo);
     else
         k_levels<false><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level,
-                                                               maxlev, e->ctr.get(), e->err.get() + kErrLevels, wd,
-                                                               blk);
+                                                               maxlev, e->ctr.get(), e->err.get() + kErrLevels, wd, bo);
     PILU_HIP(hipGetLastError());
     const int nlev = read_int(maxlev, s) + 1;
     if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
@@ -146,9 +146,11 @@ void build_block_schedule(Ext* e, bool upper, int blk, DevBuf<int>& perm, DevBuf
     while ((1 << lbits) < nlev) ++lbits;
     while ((1 << pbits) < nblk) ++pbits;
     if (lbits + pbits > 31) throw std::runtime_error("block schedule: too many blocks x levels");
+    DevBuf<int> dpos(nblk);
+    PILU_HIP(hipMemcpy(dpos.get(), pos.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
     DevBuf<unsigned> keys(n), keys_out(n);
     DevBuf<int> rows(n);
-    k_block_keys<<<grid_for(n, 256), 256, 0, s>>>(n, blk, nblk, level, upper, lbits, keys.get());
+    k_block_keys<<<grid_for(n, 256), 256, 0, s>>>(n, bo, dpos.get(), level, lbits, keys.get());
     k_iota<<<grid_for(n, 256), 256, 0, s>>>(n, rows.get());
     perm.reserve(n);
     size_t bytes = 0;
@@ -161,7 +163,7 @@ void build_block_schedule(Ext* e, bool upper, int blk, DevBuf<int>& perm, DevBuf
     std::vector<unsigned> hk(static_cast<size_t>(n));
     PILU_HIP(hipMemcpyAsync(hk.data(), keys_out.get(), sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
     PILU_HIP(hipStreamSynchronize(s));
-    std::vector<int> off, bl(static_cast<size_t>(nblk) + 1, 0);
+    std::vector<int> off, bl(static_cast<size_t>(nblk) + 1, -1);
     for (int i = 0; i < n; ++i) {
         if (i == 0 || hk[i] != hk[i - 1]) {
             const unsigned p = hk[i] >> lbits;
@@ -171,10 +173,141 @@ void build_block_schedule(Ext* e, bool upper, int blk, DevBuf<int>& perm, DevBuf
     }
     off.push_back(n);
     bl[nblk] = static_cast<int>(off.size()) - 1;
+    for (int p = nblk - 1; p >= 0; --p)
+        if (bl[p] < 0) bl[p] = bl[p + 1];  // a block without rows
     blev.reserve(bl.size());
     loff.reserve(off.size());
     PILU_HIP(hipMemcpy(blev.get(), bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(loff.get(), off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+}
+
+// Processing order of the blocks of a partition in one direction: a row
+// depends on the rows of its L (forward) or U (backward) part, so block bj
+// must run before block bi when a row of bi depends on a row of bj.  The
+// blocks are ordered by their depth in that graph (longest dependency
+// chain), ties by block id: a topological order that starts the blocks of
+// one wavefront together.  Throws if the block graph has a cycle (the
+// partition cannot be swept block by block).  Returns pos[block].
+std::vector<int> block_order(const Ext* e, const std::vector<int>& rp, const std::vector<int>& ci,
+                             const std::vector<int>& dg, bool upper) {
+    const int n = e->n, nblk = e->nblk;
+    const std::vector<int>& bo = e->h_block_of;
+    // rows grouped by block (counting sort)
+    std::vector<int> start(static_cast<size_t>(nblk) + 1, 0), rows(n);
+    for (int i = 0; i < n; ++i) ++start[bo[i] + 1];
+    for (int b = 0; b < nblk; ++b) start[b + 1] += start[b];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < n; ++i) rows[fill[bo[i]]++] = i;
+    // predecessor lists, deduplicated with a marker per block
+    std::vector<int> mark(nblk, -1), pred_ptr(static_cast<size_t>(nblk) + 1, 0), pred;
+    for (int b = 0; b < nblk; ++b) {
+        for (int k = start[b]; k < start[b + 1]; ++k) {
+            const int i = rows[k];
+            const int e0 = upper ? dg[i] + 1 : rp[i], e1 = upper ? rp[i + 1] : dg[i];
+            for (int p = e0; p < e1; ++p) {
+                const int bj = bo[ci[p]];
+                if (bj != b && mark[bj] != b) {
+                    mark[bj] = b;
+                    pred.push_back(bj);
+                }
+            }
+        }
+        pred_ptr[b + 1] = static_cast<int>(pred.size());
+    }
+    // successor lists and Kahn's algorithm, computing depths on the way
+    std::vector<int> indeg(nblk), succ_ptr(static_cast<size_t>(nblk) + 1, 0), succ(pred.size());
+    for (int b = 0; b < nblk; ++b) {
+        indeg[b] = pred_ptr[b + 1] - pred_ptr[b];
+        for (int q = pred_ptr[b]; q < pred_ptr[b + 1]; ++q) ++succ_ptr[pred[q] + 1];
+    }
+    for (int b = 0; b < nblk; ++b) succ_ptr[b + 1] += succ_ptr[b];
+    std::vector<int> sf(succ_ptr.begin(), succ_ptr.end() - 1);
+    for (int b = 0; b < nblk; ++b)
+        for (int q = pred_ptr[b]; q < pred_ptr[b + 1]; ++q) succ[sf[pred[q]]++] = b;
+    std::vector<int> depth(nblk, 0), queue;
+    queue.reserve(nblk);
+    for (int b = 0; b < nblk; ++b)
+        if (indeg[b] == 0) queue.push_back(b);
+    for (size_t h = 0; h < queue.size(); ++h) {
+        const int b = queue[h];
+        for (int q = succ_ptr[b]; q < succ_ptr[b + 1]; ++q) {
+            const int c = succ[q];
+            depth[c] = std::max(depth[c], depth[b] + 1);
+            if (--indeg[c] == 0) queue.push_back(c);
+        }
+    }
+    if (static_cast<int>(queue.size()) != nblk)
+        throw std::runtime_error("sweep partition: the block dependency graph has a cycle");
+    std::vector<int> order(nblk);
+    for (int b = 0; b < nblk; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return depth[a] < depth[c]; });
+    std::vector<int> pos(nblk);
+    for (int k = 0; k < nblk; ++k) pos[order[k]] = k;
+    return pos;
+}
+
+// Installs the partition in e->h_block_of (blocks 0..nblk-1): uploads it,
+// orders the blocks in both directions and builds both block schedules.
+// Contiguous row blocks keep ascending (forward) and descending (backward)
+// order, which k_sweep_blk relies on.
+void install_partition(Ext* e, int nblk, bool contiguous) {
+    const int n = e->n;
+    e->nblk = nblk;
+    std::vector<int> cnt(nblk, 0);
+    for (int i = 0; i < n; ++i) ++cnt[e->h_block_of[i]];
+    e->max_block_rows = *std::max_element(cnt.begin(), cnt.end());
+    if (e->max_block_rows > kMaxBlockRows)
+        throw std::runtime_error("sweep partition: a block holds more than 16384 rows");
+    e->block_of.reserve(n);
+    PILU_HIP(hipMemcpy(e->block_of.get(), e->h_block_of.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    std::vector<int> pos_l(nblk), pos_u(nblk);
+    if (contiguous) {
+        for (int b = 0; b < nblk; ++b) {
+            pos_l[b] = b;
+            pos_u[b] = nblk - 1 - b;
+        }
+    } else {
+        std::vector<int> rp(static_cast<size_t>(n) + 1), dg(n), ci(static_cast<size_t>(e->nnz));
+        PILU_HIP(hipMemcpy(rp.data(), e->rp.get(), sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+        PILU_HIP(hipMemcpy(dg.data(), e->dg.get(), sizeof(int) * dg.size(), hipMemcpyDeviceToHost));
+        PILU_HIP(hipMemcpy(ci.data(), e->ci.get(), sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+        pos_l = block_order(e, rp, ci, dg, false);
+        pos_u = block_order(e, rp, ci, dg, true);
+    }
+    build_block_schedule(e, false, pos_l, e->bperm_l, e->blev_l, e->loff_l);
+    build_block_schedule(e, true, pos_u, e->bperm_u, e->blev_u, e->loff_u);
+    ++e->part_version;  // packs are rebuilt on first use
+}
+
+// Contiguous blocks of blk rows.
+void use_blocks(Ext* e, int blk) {
+    if (blk < 64 || blk > kMaxBlockRows) throw std::runtime_error("block rows must be in [64, 16384]");
+    if (blk == e->blk_rows && e->tile[0] == 0) return;
+    e->h_block_of.resize(e->n);
+    for (int i = 0; i < e->n; ++i) e->h_block_of[i] = i / blk;
+    e->blk_rows = blk;
+    e->tile[0] = e->tile[1] = e->tile[2] = 0;
+    install_partition(e, ceil_div(e->n, blk), true);
+}
+
+// tx x ty x tz tiles of the grid declared by SetGridShape (row
+// x + nx (y + ny z)).
+void use_tiles(Ext* e, int tx, int ty, int tz) {
+    const int nx = e->grid[0], ny = e->grid[1], nz = e->grid[2];
+    if (nx <= 0) throw std::runtime_error("tiled sweeps need the grid shape (SetGridShape)");
+    if (tx < 1 || ty < 1 || tz < 1 || static_cast<long>(tx) * ty * tz > kMaxBlockRows)
+        throw std::runtime_error("tile must hold 1..16384 rows");
+    if (e->tile[0] == tx && e->tile[1] == ty && e->tile[2] == tz) return;
+    const int ntx = ceil_div(nx, tx), nty = ceil_div(ny, ty), ntz = ceil_div(nz, tz);
+    e->h_block_of.resize(e->n);
+    for (int z = 0, i = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x, ++i) e->h_block_of[i] = ((z / tz) * nty + y / ty) * ntx + x / tx;
+    e->blk_rows = 0;
+    e->tile[0] = tx;
+    e->tile[1] = ty;
+    e->tile[2] = tz;
+    install_partition(e, ntx * nty * ntz, false);
 }
 
 // Device pattern is in e->rp / e->ci: validate, locate diagonals, schedule.
@@ -221,11 +354,18 @@ void analyse(Ext* e, const int* host_rp) {
     e->factored = false;
     e->mode_l = e->mode_u = 0;
     e->blk_rows = e->nblk = 0;
+    e->tile[0] = e->tile[1] = e->tile[2] = 0;
+    e->grid[0] = e->grid[1] = e->grid[2] = 0;
+    ++e->part_version;
 }
 
-// Packed schedule-ordered factors of one direction for the current blocks.
+// Packed schedule-ordered factors of one direction for the current
+// partition: rowinfo (row, entry count) per schedule position, the slabs
+// of every (block, level) group, in-block columns encoded as
+// -(1 + schedule position within the block), and kptr (first schedule
+// position of every processed block).
 void build_pack(Ext* e, bool upper, SweepPack& pk) {
-    const int n = e->n;
+    const int n = e->n, nblk = e->nblk;
     const DevBuf<int>& perm = upper ? e->bperm_u : e->bperm_l;
     const DevBuf<int>& blev = upper ? e->blev_u : e->blev_l;
     const DevBuf<int>& loff = upper ? e->loff_u : e->loff_l;
@@ -234,11 +374,17 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
     PILU_HIP(hipMemcpy(dg.data(), e->dg.get(), sizeof(int) * dg.size(), hipMemcpyDeviceToHost));
     PILU_HIP(hipMemcpy(pm.data(), perm.get(), sizeof(int) * pm.size(), hipMemcpyDeviceToHost));
     PILU_HIP(hipMemcpy(ci.data(), e->ci.get(), sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
-    pk.h_blev.assign(static_cast<size_t>(e->nblk) + 1, 0);
+    pk.h_blev.assign(static_cast<size_t>(nblk) + 1, 0);
     PILU_HIP(hipMemcpy(pk.h_blev.data(), blev.get(), sizeof(int) * pk.h_blev.size(), hipMemcpyDeviceToHost));
-    const int ngroups = pk.h_blev[e->nblk];
+    const int ngroups = pk.h_blev[nblk];
     pk.h_loff.assign(static_cast<size_t>(ngroups) + 1, 0);
     PILU_HIP(hipMemcpy(pk.h_loff.data(), loff.get(), sizeof(int) * pk.h_loff.size(), hipMemcpyDeviceToHost));
+    // schedule position of every row and the first position of its block
+    std::vector<int> kinv(n), kptr(static_cast<size_t>(nblk) + 1), kb0(n);
+    for (int k = 0; k < n; ++k) kinv[pm[k]] = k;
+    for (int p = 0; p <= nblk; ++p) kptr[p] = pk.h_loff[pk.h_blev[p]];
+    for (int p = 0; p < nblk; ++p)
+        for (int k = kptr[p]; k < kptr[p + 1]; ++k) kb0[pm[k]] = kptr[p];
     pk.h_base.assign(ngroups, 0);
     pk.h_width.assign(ngroups, 0);
     std::vector<int> rowinfo(2 * static_cast<size_t>(n));
@@ -259,6 +405,7 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
         total += static_cast<long>(w) * (pk.h_loff[g + 1] - pk.h_loff[g]);
         if (total >= (1L << 31)) throw std::runtime_error("packed sweep: too many entries");
     }
+    const std::vector<int>& bo = e->h_block_of;
     std::vector<int> pcol(std::max<long>(total, 1), 0), psrc(std::max<long>(total, 1), -1);
     for (int g = 0; g < ngroups; ++g) {
         const int count = pk.h_loff[g + 1] - pk.h_loff[g];
@@ -268,7 +415,8 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
             const int e0 = upper ? dg[row] + 1 : rp[row];
             for (int q = 0; q < cnt; ++q) {
                 const long pos = pk.h_base[g] + static_cast<long>(q) * count + (k - pk.h_loff[g]);
-                pcol[pos] = ci[e0 + q];
+                const int c = ci[e0 + q];
+                pcol[pos] = bo[c] == bo[row] ? -1 - (kinv[c] - kb0[row]) : c;
                 psrc[pos] = e0 + q;
             }
         }
@@ -278,63 +426,47 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
     pk.rowinfo.resize(rowinfo.size());
     pk.pcol.resize(pcol.size());
     pk.psrc.resize(psrc.size());
+    pk.kptr.resize(kptr.size());
     PILU_HIP(hipMemcpy(pk.rowinfo.get(), rowinfo.data(), sizeof(int) * rowinfo.size(), hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(pk.pcol.get(), pcol.data(), sizeof(int) * pcol.size(), hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(pk.psrc.get(), psrc.data(), sizeof(int) * psrc.size(), hipMemcpyHostToDevice));
-    pk.blk = e->blk_rows;
+    PILU_HIP(hipMemcpy(pk.kptr.get(), kptr.data(), sizeof(int) * kptr.size(), hipMemcpyHostToDevice));
+    pk.version = e->part_version;
     pk.pass_g = 0;
+    pk.factors = -1;
 }
 
-// Pass list of a pack for G lanes per row (512 / G rows per pass).
+// Pass list of a pack for G lanes per row: every level group in chunks of
+// 512 / G rows, each block's list padded to a multiple of kPpUnroll.
 void build_passes(Ext* e, SweepPack& pk, int G) {
     if (pk.pass_g == G) return;
     const int per = 512 / G;
-    std::vector<PackPass> passes;
-    std::vector<int> ptr(static_cast<size_t>(e->nblk) + 1, 0);
-    for (int p = 0; p < e->nblk; ++p) {
-        ptr[p] = static_cast<int>(passes.size());
-        for (int g = pk.h_blev[p]; g < pk.h_blev[p + 1]; ++g) {
-            const int lo = pk.h_loff[g], hi = pk.h_loff[g + 1];
-            for (int c = lo; c < hi; c += per) {
-                PackPass r{};
-                r.k_lo = c;
-                r.k_hi = std::min(c + per, hi);
-                r.gstart = lo;
-                r.base = pk.h_base[g];
-                r.count = hi - lo;
-                r.width = pk.h_width[g];
-                r.bar = c + per >= hi ? 1 : 0;
-                passes.push_back(r);
-            }
-        }
-    }
-    ptr[e->nblk] = static_cast<int>(passes.size());
-    // Compact descriptors for the pipelined sweep, each block's list padded
-    // with empty passes to a multiple of kPpUnroll.
     std::vector<PassDesc> desc;
     std::vector<int> dptr(static_cast<size_t>(e->nblk) + 1, 0);
     pk.max_np = 0;
     for (int p = 0; p < e->nblk; ++p) {
         dptr[p] = static_cast<int>(desc.size());
-        for (int i = ptr[p]; i < ptr[p + 1]; ++i) {
-            const PackPass& r = passes[i];
-            const int j0 = r.k_lo - r.gstart;
-            if (j0 >= (1 << 16) || r.width >= (1 << 15))
+        int k_any = -1;
+        for (int g = pk.h_blev[p]; g < pk.h_blev[p + 1]; ++g) {
+            const int lo = pk.h_loff[g], hi = pk.h_loff[g + 1], count = hi - lo, width = pk.h_width[g];
+            if (count >= (1 << 16) || width >= (1 << 15))
                 throw std::runtime_error("packed sweep: level group too large for the pass descriptor");
-            // every slab position and schedule position the pass can touch
-            // lies inside the pack
-            const int nact = std::min(512 / G, r.count - j0);
-            if (nact <= 0 || r.k_lo + nact > e->n ||
-                (r.width > 0 && r.base + static_cast<long>(r.width - 1) * r.count + j0 + nact > std::max<long>(pk.ne, 1)))
-                throw std::runtime_error("packed sweep: pass descriptor out of range");
-            PassDesc d;
-            d.k_lo = r.k_lo;
-            d.base = r.base;
-            d.cb = r.count | (r.bar ? static_cast<int>(0x80000000u) : 0);
-            d.jw = j0 | (r.width << 16);
-            desc.push_back(d);
+            for (int c = lo; c < hi; c += per) {
+                const int nact = std::min(per, hi - c);
+                // every slab and schedule position the pass can touch lies in the pack
+                if (c + nact > e->n ||
+                    (width > 0 && pk.h_base[g] + static_cast<long>(width - 1) * count + (c - lo) + nact > pk.ne))
+                    throw std::runtime_error("packed sweep: pass descriptor out of range");
+                PassDesc d;
+                d.k_lo = c;
+                d.base = pk.h_base[g];
+                d.cb = count | (c + per >= hi ? static_cast<int>(0x80000000u) : 0);
+                d.jw = (c - lo) | (width << 16);
+                desc.push_back(d);
+                k_any = lo;
+            }
         }
-        const int k_any = ptr[p + 1] > ptr[p] ? passes[ptr[p]].k_lo : 0;
+        if (k_any < 0) k_any = 0;
         while ((desc.size() - dptr[p]) % kPpUnroll) desc.push_back(PassDesc{k_any, 0, 0, 0});
         pk.max_np = std::max(pk.max_np, static_cast<int>(desc.size()) - dptr[p]);
     }
@@ -344,16 +476,12 @@ void build_passes(Ext* e, SweepPack& pk, int G) {
     pk.desc_ptr.resize(dptr.size());
     PILU_HIP(hipMemcpy(pk.pdesc.get(), desc.data(), sizeof(PassDesc) * desc.size(), hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(pk.desc_ptr.get(), dptr.data(), sizeof(int) * dptr.size(), hipMemcpyHostToDevice));
-    pk.pass_ptr.resize(ptr.size());
-    pk.passes.resize(sizeof(PackPass) * std::max<size_t>(passes.size(), 1));
-    PILU_HIP(hipMemcpy(pk.pass_ptr.get(), ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
-    PILU_HIP(hipMemcpy(pk.passes.get(), passes.data(), sizeof(PackPass) * passes.size(), hipMemcpyHostToDevice));
     pk.pass_g = G;
 }
 
 // Refresh the packed values from the current factors.
 void gather_pack(Ext* e, SweepPack& pk, bool upper) {
-    if (pk.ne == 0 || pk.blk != e->blk_rows) return;
+    if (pk.ne == 0 || pk.version != e->part_version) return;
     hipStream_t s = e->stream;
     const int grid = 2048;
     if (e->precision == 32) {
@@ -370,27 +498,17 @@ void gather_pack(Ext* e, SweepPack& pk, bool upper) {
                                                            e->dg.get(), upper ? pk.pdiag64.get() : nullptr);
     }
     PILU_HIP(hipGetLastError());
+    pk.factors = e->factor_version;
 }
 
 void ensure_packs(Ext* e) {
-    if (e->blk_rows <= 0) throw std::runtime_error("packed sweep needs block rows");
-    if (e->pack_l.blk != e->blk_rows) build_pack(e, false, e->pack_l);
-    if (e->pack_u.blk != e->blk_rows) build_pack(e, true, e->pack_u);
+    if (e->nblk <= 0) throw std::runtime_error("packed sweep needs a block partition");
+    if (e->pack_l.version != e->part_version) build_pack(e, false, e->pack_l);
+    if (e->pack_u.version != e->part_version) build_pack(e, true, e->pack_u);
     if (e->factored) {
-        gather_pack(e, e->pack_l, false);
-        gather_pack(e, e->pack_u, true);
+        if (e->pack_l.factors != e->factor_version) gather_pack(e, e->pack_l, false);
+        if (e->pack_u.factors != e->factor_version) gather_pack(e, e->pack_u, true);
     }
-}
-
-// Blocks of blk rows for the block-local sweeps (both directions).
-void use_blocks(Ext* e, int blk) {
-    if (blk == e->blk_rows) return;
-    if (blk < 64 || blk > 16384) throw std::runtime_error("block rows must be in [64, 16384]");
-    build_block_schedule(e, false, blk, e->bperm_l, e->blev_l, e->loff_l);
-    build_block_schedule(e, true, blk, e->bperm_u, e->blev_u, e->loff_u);
-    e->blk_rows = blk;
-    e->pack_l.blk = e->pack_u.blk = 0;  // packs are rebuilt on first use
-    e->nblk = ceil_div(e->n, blk);
 }
 
 template <int G, typename T, bool UPPER>
@@ -430,32 +548,17 @@ void sweep_blk_dispatch(int lanes, Ext* e, const T* val, const double* b, double
     }
 }
 
-template <int G, int S, typename T, bool UPPER>
-void sweep_pk_one(Ext* e, const T* pval, const T* pdiag, const double* b, double* out, double* reset, const int* gate,
-                  int* ctr) {
-    SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
-    build_passes(e, pk, G);
-    const size_t lds = sizeof(double) * static_cast<size_t>(e->blk_rows);
-    auto kern = k_sweep_pk<G, S, T, UPPER>;
-    PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 static_cast<int>(lds)));
-    kern<<<e->nblk, 512, lds, e->stream>>>(e->n, e->blk_rows, e->nblk, pk.pass_ptr.get(),
-                                          reinterpret_cast<const PackPass*>(pk.passes.get()),
-                                          reinterpret_cast<const int2*>(pk.rowinfo.get()), pk.pcol.get(), pval, pdiag,
-                                          b, out, reset, gate, ctr, e->err.get() + kErrSweep,
-                                          watchdog_ticks(kWatchdogSeconds));
-}
-
-// LDS of the pipelined packed sweep: the block's x (b on entry) and its
-// pass descriptors.
+// LDS of the packed sweep: the block's x (b on entry) and its pass list.
 size_t pp_lds_bytes(const Ext* e, const SweepPack& pk) {
-    return sizeof(double) * static_cast<size_t>(e->blk_rows) + sizeof(PassDesc) * static_cast<size_t>(pk.max_np);
+    return sizeof(double) * static_cast<size_t>(e->max_block_rows) +
+           sizeof(PassDesc) * static_cast<size_t>(pk.max_np);
 }
 constexpr size_t kMaxLds = 160 * 1024;
 
-// True when the pipelined packed sweep with G lanes per row fits the LDS in
-// both directions for the current blocks.
+// True when the packed sweep with G lanes per row fits the LDS in both
+// directions for the current partition.
 bool pp_fits(Ext* e, int G) {
+    ensure_packs(e);
     build_passes(e, e->pack_l, G);
     build_passes(e, e->pack_u, G);
     return pp_lds_bytes(e, e->pack_l) <= kMaxLds && pp_lds_bytes(e, e->pack_u) <= kMaxLds;
@@ -467,19 +570,19 @@ void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double
     SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
     build_passes(e, pk, G);
     const size_t lds = pp_lds_bytes(e, pk);
-    if (lds > kMaxLds) throw std::runtime_error("pipelined packed sweep: block and pass list exceed the LDS");
+    if (lds > kMaxLds) throw std::runtime_error("packed sweep: block and pass list exceed the LDS");
     auto kern = k_sweep_pp<G, S, kPpUnroll, kPpUnroll / 2, T, UPPER>;
     PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
-    kern<<<e->nblk, 512, lds, e->stream>>>(e->n, e->blk_rows, e->nblk, pk.desc_ptr.get(),
+    kern<<<e->nblk, 512, lds, e->stream>>>(pk.kptr.get(), pk.desc_ptr.get(),
                                           reinterpret_cast<const PassDesc*>(pk.pdesc.get()),
                                           reinterpret_cast<const int2*>(pk.rowinfo.get()), pk.pcol.get(), pval, pdiag,
                                           b, out, reset, gate, ctr, e->err.get() + kErrSweep,
-                                          watchdog_ticks(kWatchdogSeconds));
+                                          watchdog_ticks(kWatchdogSeconds), e->max_block_rows);
 }
 
 template <int G, typename T, bool UPPER>
-void sweep_pk_s(Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+void sweep_pp_s(Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
     SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
     const T* pval;
     const T* pdiag;
@@ -491,37 +594,32 @@ void sweep_pk_s(Ext* e, const double* b, double* out, double* reset, const int* 
         pdiag = UPPER ? pk.pdiag64.get() : nullptr;
     }
     const int need = (pk.max_w + G - 1) / G;
-    const bool pipelined = (UPPER ? e->mode_u : e->mode_l) == 3;
-    if (pipelined) {
-        if (need <= 1) sweep_pp_one<G, 1, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
-        else if (need <= 2) sweep_pp_one<G, 2, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
-        else sweep_pp_one<G, 4, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
-        return;
-    }
-    if (need <= 1) sweep_pk_one<G, 1, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
-    else if (need <= 2) sweep_pk_one<G, 2, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
-    else sweep_pk_one<G, 4, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+    if (need <= 1) sweep_pp_one<G, 1, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+    else if (need <= 2) sweep_pp_one<G, 2, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+    else sweep_pp_one<G, 4, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
 }
 
 template <typename T, bool UPPER>
-void sweep_pk_dispatch(int lanes, Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+void sweep_pp_dispatch(int lanes, Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
     switch (lanes) {
-        case 1: sweep_pk_s<1, T, UPPER>(e, b, out, reset, gate, ctr); break;
-        case 2: sweep_pk_s<2, T, UPPER>(e, b, out, reset, gate, ctr); break;
-        case 4: sweep_pk_s<4, T, UPPER>(e, b, out, reset, gate, ctr); break;
-        case 8: sweep_pk_s<8, T, UPPER>(e, b, out, reset, gate, ctr); break;
-        default: sweep_pk_s<16, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 1: sweep_pp_s<1, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 2: sweep_pp_s<2, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 4: sweep_pp_s<4, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 8: sweep_pp_s<8, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        default: sweep_pp_s<16, T, UPPER>(e, b, out, reset, gate, ctr); break;
     }
 }
 
 template <typename T, bool UPPER>
 void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
                     const int* gate, int* ctr) {
-    if ((UPPER ? e->mode_u : e->mode_l) >= 2 && e->blk_rows > 0) {
-        sweep_pk_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
+    const int mode = UPPER ? e->mode_u : e->mode_l;
+    if (mode == 2 && e->nblk > 0) {
+        ensure_packs(e);
+        sweep_pp_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
         return;
     }
-    if ((UPPER ? e->mode_u : e->mode_l) == 1 && e->blk_rows > 0) {
+    if (mode == 1 && e->blk_rows > 0) {
         sweep_blk_dispatch<T, UPPER>(lanes, e, val, b, out, reset, gate, ctr);
         return;
     }
@@ -573,6 +671,7 @@ bool run_factor(Ext* e) {
     if (err & kErrTimeout) throw std::runtime_error("device factorization: dependency wait timed out (watchdog)");
     if (err & 2) throw std::runtime_error("device factorization: row exceeded the LDS staging capacity");
     e->factored = true;
+    ++e->factor_version;
     gather_pack(e, e->pack_l, false);
     gather_pack(e, e->pack_u, true);
     return (err & 1) == 0;
@@ -783,17 +882,27 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     static const int kLanes[7] = {1, 2, 4, 8, 16, 32, 64};
     static const int kBlkLanes[4] = {4, 8, 16, 32};
     static const int kPackLanes[4] = {2, 4, 8, 16};
-    static const int kBlocks[3] = {4096, 8192, 16384};
+    struct Part { int blk, tx, ty, tz; };
+    std::vector<Part> parts;
+    for (int blk : {4096, 8192, 16384})
+        if (blk <= e->n) parts.push_back({blk, 0, 0, 0});
+    if (e->grid[0] > 0) {
+        static const int kTiles[5][3] = {{16, 16, 32}, {32, 16, 16}, {16, 16, 16}, {32, 32, 8}, {16, 16, 64}};
+        for (const auto& t : kTiles)
+            if (t[0] <= e->grid[0] && t[1] <= e->grid[1] && t[2] <= e->grid[2] &&
+                static_cast<long>(t[0]) * t[1] * t[2] < e->n)
+                parts.push_back({0, t[0], t[1], t[2]});
+    }
     pilu::DevBuf<double> b(e->n), x(e->n);
     pilu::k_fill_u64<<<pilu::grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(b.get()),
                                                                    0x3FF0000000000000ull);  // 1.0
     hipEvent_t ev[3];
     for (auto& v : ev) PILU_HIP(hipEventCreate(&v));
     int rows = 0;
-    auto record = [&](int mode, int blk, int lanes, double tl, double tu) {
+    auto record = [&](int mode, const Part& pt, int lanes, double tl, double tu) {
         if (table && rows < cap) {
-            double* r = table + 5 * rows;
-            r[0] = mode; r[1] = blk; r[2] = lanes; r[3] = tl; r[4] = tu;
+            double* r = table + 8 * rows;
+            r[0] = mode; r[1] = pt.blk; r[2] = pt.tx; r[3] = pt.ty; r[4] = pt.tz; r[5] = lanes; r[6] = tl; r[7] = tu;
         }
         ++rows;
     };
@@ -804,49 +913,43 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     for (int g : kLanes) {
         double tl, tu;
         pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
-        record(0, 0, g, tl, tu);
+        record(0, Part{0, 0, 0, 0}, g, tl, tu);
         if (tl < lev_best[0]) { lev_best[0] = tl; lev_lanes[0] = g; }
         if (tu < lev_best[1]) { lev_best[1] = tu; lev_lanes[1] = g; }
     }
-    // block-local, per block size
+    // block-local sweeps, per partition
     double best_total = lev_best[0] + lev_best[1];
-    int pick_mode[2] = {0, 0}, pick_lanes[2] = {lev_lanes[0], lev_lanes[1]}, pick_blk = 0;
-    for (int blk : kBlocks) {
-        if (blk > e->n) break;
-        pilu::use_blocks(e, blk);
-        e->mode_l = e->mode_u = 1;
+    int pick_mode[2] = {0, 0}, pick_lanes[2] = {lev_lanes[0], lev_lanes[1]};
+    int pick = -1;
+    for (size_t ip = 0; ip < parts.size(); ++ip) {
+        const Part& pt = parts[ip];
+        if (pt.blk) pilu::use_blocks(e, pt.blk);
+        else pilu::use_tiles(e, pt.tx, pt.ty, pt.tz);
         double bb[2] = {1e300, 1e300};
-        int bl[2] = {8, 8};
-        for (int g : kBlkLanes) {
-            double tl, tu;
-            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
-            record(1, blk, g, tl, tu);
-            if (tl < bb[0]) { bb[0] = tl; bl[0] = g; }
-            if (tu < bb[1]) { bb[1] = tu; bl[1] = g; }
+        int bl[2] = {8, 8}, bm[2] = {1, 1};
+        if (pt.blk) {
+            e->mode_l = e->mode_u = 1;
+            for (int g : kBlkLanes) {
+                double tl, tu;
+                pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+                record(1, pt, g, tl, tu);
+                if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 1; }
+                if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 1; }
+            }
         }
-        int bm[2] = {1, 1};
-        pilu::ensure_packs(e);
         e->mode_l = e->mode_u = 2;
         for (int g : kPackLanes) {
+            if (!pilu::pp_fits(e, g)) continue;
             double tl, tu;
             pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
-            record(2, blk, g, tl, tu);
+            record(2, pt, g, tl, tu);
             if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 2; }
             if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 2; }
-        }
-        e->mode_l = e->mode_u = 3;
-        for (int g : kPackLanes) {
-            double tl, tu;
-            if (!pilu::pp_fits(e, g)) continue;
-            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
-            record(3, blk, g, tl, tu);
-            if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 3; }
-            if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 3; }
         }
         const double tot = std::min(bb[0], lev_best[0]) + std::min(bb[1], lev_best[1]);
         if (tot < best_total) {
             best_total = tot;
-            pick_blk = blk;
+            pick = static_cast<int>(ip);
             for (int d = 0; d < 2; ++d) {
                 pick_mode[d] = bb[d] < lev_best[d] ? bm[d] : 0;
                 pick_lanes[d] = bb[d] < lev_best[d] ? bl[d] : lev_lanes[d];
@@ -855,23 +958,47 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     }
     for (auto& v : ev) (void)hipEventDestroy(v);
     pilu::check_sweep_errors(e);
-    if (pick_blk > 0) pilu::use_blocks(e, pick_blk);
-    if (pick_mode[0] >= 2 || pick_mode[1] >= 2) pilu::ensure_packs(e);
+    if (pick >= 0) {
+        const Part& pt = parts[pick];
+        if (pt.blk) pilu::use_blocks(e, pt.blk);
+        else pilu::use_tiles(e, pt.tx, pt.ty, pt.tz);
+    }
     e->mode_l = pick_mode[0];
     e->mode_u = pick_mode[1];
     e->lanes_l = pick_lanes[0];
     e->lanes_u = pick_lanes[1];
+    if (pick_mode[0] == 2 || pick_mode[1] == 2) pilu::ensure_packs(e);
     pilu::show_algorithm(this);
     return rows;
 }
 
 void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
     Ext* e = pilu::need_setup(ext_);
-    if (mode_l < 0 || mode_l > 3 || mode_u < 0 || mode_u > 3) throw std::runtime_error("sweep mode is 0, 1, 2 or 3");
+    if (mode_l < 0 || mode_l > 2 || mode_u < 0 || mode_u > 2) throw std::runtime_error("sweep mode is 0, 1 or 2");
     if (mode_l > 0 || mode_u > 0) pilu::use_blocks(e, blk_rows);
-    if (mode_l >= 2 || mode_u >= 2) pilu::ensure_packs(e);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
+    if (mode_l == 2 || mode_u == 2) pilu::ensure_packs(e);
+    pilu::show_algorithm(this);
+}
+
+void IluSolver::SetGridShape(int nx, int ny, int nz) {
+    Ext* e = pilu::need_setup(ext_);
+    if (nx <= 0 || ny <= 0 || nz <= 0 || static_cast<long>(nx) * ny * nz != e->n)
+        throw std::runtime_error("grid shape must satisfy nx * ny * nz == n");
+    e->grid[0] = nx;
+    e->grid[1] = ny;
+    e->grid[2] = nz;
+}
+
+void IluSolver::SetSweepTiles(int mode_l, int mode_u, int tx, int ty, int tz) {
+    Ext* e = pilu::need_setup(ext_);
+    if (mode_l == 1 || mode_u == 1 || mode_l < 0 || mode_l > 2 || mode_u < 0 || mode_u > 2)
+        throw std::runtime_error("tiled sweeps use mode 0 or 2");
+    pilu::use_tiles(e, tx, ty, tz);
+    e->mode_l = mode_l;
+    e->mode_u = mode_u;
+    if (mode_l == 2 || mode_u == 2) pilu::ensure_packs(e);
     pilu::show_algorithm(this);
 }
 
@@ -887,13 +1014,17 @@ std::string IluSolver::DescribeSchedule() const {
     };
     std::snprintf(buf, sizeof(buf),
                   "{\"n\": %d, \"nnz\": %ld, \"max_row\": %d, \"precision\": %d, \"levels_l\": %d, "
-                  "\"levels_u\": %d, \"blk_rows\": %d, \"blocks\": %d, \"mode_l\": %d, \"lanes_l\": %d, "
-                  "\"mode_u\": %d, \"lanes_u\": %d, ",
-                  e->n, e->nnz, e->max_row, e->precision, e->nlev_l, e->nlev_u, e->blk_rows, e->nblk, e->mode_l,
+                  "\"levels_u\": %d, \"grid\": [%d, %d, %d], \"blk_rows\": %d, \"tile\": [%d, %d, %d], "
+                  "\"blocks\": %d, \"max_block_rows\": %d, \"mode_l\": %d, \"lanes_l\": %d, \"mode_u\": %d, "
+                  "\"lanes_u\": %d, ",
+                  e->n, e->nnz, e->max_row, e->precision, e->nlev_l, e->nlev_u, e->grid[0], e->grid[1], e->grid[2],
+                  e->blk_rows, e->tile[0], e->tile[1], e->tile[2], e->nblk, e->max_block_rows, e->mode_l,
                   pilu::effective_lanes(e, false), e->mode_u, pilu::effective_lanes(e, true));
     std::string js(buf);
-    js += "\"pack_l\": " + (e->pack_l.blk ? pack(e->pack_l) : std::string("null"));
-    js += ", \"pack_u\": " + (e->pack_u.blk ? pack(e->pack_u) : std::string("null")) + "}";
+    const bool live_l = e->nblk > 0 && e->pack_l.version == e->part_version;
+    const bool live_u = e->nblk > 0 && e->pack_u.version == e->part_version;
+    js += "\"pack_l\": " + (live_l ? pack(e->pack_l) : std::string("null"));
+    js += ", \"pack_u\": " + (live_u ? pack(e->pack_u) : std::string("null")) + "}";
     return js;
 }
 
@@ -904,6 +1035,9 @@ void IluSolver::GetSweepConfig(int* cfg) const {
     cfg[2] = e ? e->mode_u : 0;
     cfg[3] = e ? pilu::effective_lanes(e, true) : 0;
     cfg[4] = e ? e->blk_rows : 0;
+    cfg[5] = e ? e->tile[0] : 0;
+    cfg[6] = e ? e->tile[1] : 0;
+    cfg[7] = e ? e->tile[2] : 0;
 }
 
 void IluSolver::SetFactorPrecision(int bits) {

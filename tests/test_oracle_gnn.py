@@ -47,3 +47,16 @@ def test_features_are_scale_free():
     a, feat, w, _, _ = small_problem(2)
     feat2, w2 = G.features(a.row_ptr, a.col_idx, 7.5 * a.value)
     assert np.allclose(feat, feat2, rtol=1e-6) and np.allclose(w, w2, rtol=1e-6)
+
+
+def test_training_worker_builds_consistent_transpose():
+    """train_gnn's generation worker: the GRF matrix, its right-hand side
+    and a transpose map whose entries point at the transposed positions."""
+    from par_ilu0_cpp_amd.train_gnn import _make
+
+    seed, a, b, (rpt, cit, tmap) = _make((6, 1.5, 8.0, 3, True))
+    assert seed == 3 and b.shape == (a.n,)
+    rows = np.repeat(np.arange(a.n), np.diff(a.row_ptr))
+    rows_t = np.repeat(np.arange(a.n), np.diff(rpt))
+    # entry k of A^T is (rows_t[k], cit[k]) and equals entry tmap[k] of A transposed
+    assert np.array_equal(a.col_idx[tmap], rows_t) and np.array_equal(rows[tmap], cit)
