@@ -125,6 +125,108 @@ __global__ __launch_bounds__(kVecThreads) void k_mgs(int n, const double* __rest
     if (threadIdx.x == 0) part_out[blockIdx.x] = tot;
 }
 
+// ---------------------------------------------------------------------
+// The whole modified Gram-Schmidt of step i in one launch (src/gmres.cpp:
+// 219-233): workgroup g keeps rows [g*chunk, (g+1)*chunk) of w in
+// registers (R per thread) and, for j = 0..i,
+//   w -= h_j v_j;  partial of (v_{j+1}, w), or of (w, w) after the last j;
+// every workgroup publishes its partial as a value-flag and sums all of
+// them in index order, so every workgroup holds the same h_{j+1}.  v_{j+1}
+// stays in registers for the next term: each basis vector is read once.
+// h_0 comes from the SpMV's kVecBlocks partials as in the per-term path.
+// Finally w is scaled by 1 / ||w|| unless ||w|| <= 1e-16 (the breakdown
+// of src/gmres.cpp:228-233) and ||w|| is left in *tnorm.  The grid must be
+// co-resident (cooperative launch); partial slots come in two generations
+// (step parity), each step re-arming the other one.  Every wait carries the
+// watchdog.
+// ---------------------------------------------------------------------
+constexpr int kMgsThreads = 512;
+
+__device__ double block_sum512(double v) {
+    __shared__ double s_part[kMgsThreads / 64];
+    __shared__ double s_total;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kMgsThreads / 64; ++k) t += s_part[k];
+        s_total = t;
+    }
+    __syncthreads();
+    return s_total;
+}
+
+template <int R>
+__global__ __launch_bounds__(kMgsThreads) void k_mgs_all(int n, int i, int m, int chunk, const double* __restrict__ vv,
+                                                         double* w, const double* part0, double* pbuf, double* col,
+                                                         double* tnorm, const GmresCtl* ctl, int* err,
+                                                         uint64_t wd_limit) {
+    if (ctl->step_active == 0) return;
+    const int nwg = gridDim.x, wg = blockIdx.x;
+    double* pb = pbuf + static_cast<long>(i & 1) * (m + 1) * nwg;
+    double* pnext = pbuf + static_cast<long>((i + 1) & 1) * (m + 1) * nwg;
+    for (int j = threadIdx.x; j <= m; j += kMgsThreads)
+        publish_raw(reinterpret_cast<uint64_t*>(pnext + static_cast<long>(j) * nwg + wg), kPending);
+    const Watchdog wd(wd_limit);
+    bool timed_out = false;
+    const long r0 = static_cast<long>(wg) * chunk;
+    const long r1 = r0 + chunk < n ? r0 + chunk : n;
+    double wr[R], vn[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const long r = r0 + threadIdx.x + static_cast<long>(k) * kMgsThreads;
+        wr[k] = r < r1 ? w[r] : 0.0;
+        vn[k] = r < r1 ? vv[r] : 0.0;
+    }
+    double h = 0.0;
+    {
+        double v = 0.0;
+        for (int k = threadIdx.x; k < kVecBlocks; k += kMgsThreads) v += part0[k];
+        h = block_sum512(v);
+    }
+    for (int j = 0; j <= i; ++j) {
+        if (wg == 0 && threadIdx.x == 0) col[j] = h;
+        const double* vnext = j < i ? vv + static_cast<long>(j + 1) * n : nullptr;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const long r = r0 + threadIdx.x + static_cast<long>(k) * kMgsThreads;
+            wr[k] = fma(-h, vn[k], wr[k]);
+            if (vnext) vn[k] = r < r1 ? vnext[r] : 0.0;
+            acc = vnext ? fma(vn[k], wr[k], acc) : fma(wr[k], wr[k], acc);
+        }
+        const double tot = block_sum512(acc);
+        double* slot = pb + static_cast<long>(j) * nwg;
+        if (threadIdx.x == 0) publish(slot + wg, tot);
+        double v = 0.0;
+        if (threadIdx.x < nwg) {
+            uint64_t bits;
+            while ((bits = peek_bits(slot + threadIdx.x)) == kPending) {
+                if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
+                spin_pause();
+            }
+            v = __longlong_as_double(bits);
+        }
+        h = block_sum512(v);
+    }
+    const double t = std::sqrt(h);
+    if (wg == 0 && threadIdx.x == 0) *tnorm = t;
+    const double a = 1.0 / t;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const long r = r0 + threadIdx.x + static_cast<long>(k) * kMgsThreads;
+        if (r < r1) w[r] = t <= 1e-16 ? wr[k] : wr[k] * a;
+    }
+    if (timed_out && threadIdx.x == 0) atomicOr(err, kErrTimeout);
+}
+
+__global__ void k_fill_pending(long n, uint64_t* p) {
+    const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < n; k += stride) p[k] = kPending;
+}
+
 // v *= ctl->inv when the current cycle/step calls for it.
 __global__ __launch_bounds__(kVecThreads) void k_scale(int n, double* v, const GmresCtl* ctl, bool step_gate) {
     if (step_gate ? (ctl->step_active == 0 || ctl->breakdown) : ctl->cycle_active == 0) return;
@@ -167,9 +269,9 @@ __global__ void k_step_gate(GmresCtl* ctl, int i) {
 
 // End of an Arnoldi step (src/gmres.cpp:226-262): h_{i+1,i} = ||w||,
 // breakdown test, earlier rotations on column i, new rotation, estimate.
-__global__ void k_arnoldi_finish(GmresCtl* ctl, const double* part, double* hh, double* hist) {
+__global__ void k_arnoldi_finish(GmresCtl* ctl, const double* part, const double* tnorm, double* hh, double* hist) {
     if (ctl->step_active == 0) return;
-    const double t = std::sqrt(sum_partials(part));
+    const double t = tnorm ? *tnorm : std::sqrt(sum_partials(part));
     if (threadIdx.x != 0) return;
     const int im = ctl->im, im1 = im + 1;
     const int i = ctl->i, i1 = i + 1;
@@ -264,7 +366,7 @@ __global__ void k_sqrt_sum(const double* part, double* out) {
 
 struct lljbash::PreconditionedGmres::Ext {
     int n = 0, m = 0;
-    pilu::DevBuf<double> vv, z, hh, part, hist, scratch;
+    pilu::DevBuf<double> vv, z, hh, part, hist, scratch, pbuf;
     pilu::DevBuf<int> flags;
     pilu::DevBuf<pilu::GmresCtl> ctl;
     pilu::DevBuf<int> arp, aci;
@@ -306,6 +408,74 @@ void spmv(int lanes, hipStream_t s, int n, const int* rp, const int* ci, const d
     }
 }
 
+inline int grid_for_n(long n, int per) { return static_cast<int>(std::max<long>(1, std::min<long>((n + per - 1) / per, 65535))); }
+
+// Launch shape of k_mgs_all: one co-resident workgroup per compute unit,
+// rows per thread R a power of two up to 16 (32 would spill registers);
+// nwg == 0 when n is too large (the per-term kernels are used then).
+struct MgsPlan {
+    int nwg = 0, chunk = 0, r = 0;
+};
+
+template <int R>
+bool mgs_fits(int nwg) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pilu::k_mgs_all<R>),
+                                                     pilu::kMgsThreads, 0) != hipSuccess)
+        return false;
+    return per_cu >= 1 && nwg <= pilu::kMgsThreads;
+}
+
+MgsPlan plan_mgs(int n) {
+    MgsPlan p;
+    static int cus = -1;
+    if (cus < 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 0;
+    }
+    if (cus <= 0 || n <= 0) return p;
+    const int chunk = (n + cus - 1) / cus;
+    int r = 1;
+    while (r < 16 && r * pilu::kMgsThreads < chunk) r <<= 1;
+    if (r * pilu::kMgsThreads < chunk) return p;
+    bool ok = false;
+    switch (r) {
+        case 1: ok = mgs_fits<1>(cus); break;
+        case 2: ok = mgs_fits<2>(cus); break;
+        case 4: ok = mgs_fits<4>(cus); break;
+        case 8: ok = mgs_fits<8>(cus); break;
+        default: ok = mgs_fits<16>(cus); break;
+    }
+    if (!ok) return p;
+    p.nwg = cus;
+    p.chunk = chunk;
+    p.r = r;
+    return p;
+}
+
+template <int R>
+void launch_mgs_r(const MgsPlan& p, hipStream_t s, int n, int i, int m, const double* vv, double* w,
+                  const double* part0, double* pbuf, double* col, double* tnorm, const pilu::GmresCtl* ctl, int* err,
+                  uint64_t wd) {
+    int chunk = p.chunk;
+    void* args[] = {&n, &i, &m, &chunk, &vv, &w, &part0, &pbuf, &col, &tnorm, &ctl, &err, &wd};
+    PILU_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pilu::k_mgs_all<R>), dim3(p.nwg),
+                                        dim3(pilu::kMgsThreads), args, 0, s));
+}
+
+void launch_mgs_all(const MgsPlan& p, hipStream_t s, int n, int i, int m, const double* vv, double* w,
+                    const double* part0, double* pbuf, double* col, double* tnorm, const pilu::GmresCtl* ctl,
+                    int* err, uint64_t wd) {
+    switch (p.r) {
+        case 1: launch_mgs_r<1>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        case 2: launch_mgs_r<2>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        case 4: launch_mgs_r<4>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        case 8: launch_mgs_r<8>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        default: launch_mgs_r<16>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+    }
+}
+
 struct TimedPair {
     hipEvent_t a, b;
     int kind;  // 0 = mv, 1 = precon
@@ -324,7 +494,7 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
     e->vv.reserve(static_cast<size_t>(im1) * n);
     e->z.reserve(static_cast<size_t>(m) * n);
     e->hh.reserve(static_cast<size_t>(im1) * (m + 3));
-    e->part.reserve(2 * pilu::kVecBlocks);
+    e->part.reserve(2 * pilu::kVecBlocks + 8);
     e->hist.reserve(static_cast<size_t>(prm.max_iterations) + 2);
     e->scratch.reserve(n);
     e->ctl.reserve(1);
@@ -336,6 +506,14 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
     }
     const double avg = n > 0 ? static_cast<double>(nnz) / n : 1.0;
     const int lanes = pilu::lanes_for_avg(avg);
+    // One-launch Gram-Schmidt when each of the co-resident workgroups can
+    // keep its rows of w in registers.
+    const MgsPlan mgs = plan_mgs(n);
+    double* pbuf = nullptr;
+    if (mgs.nwg > 0) {
+        e->pbuf.reserve(static_cast<size_t>(2) * (m + 1) * mgs.nwg);
+        pbuf = e->pbuf.get();
+    }
 
     // Validate A before any kernel indexes through it.
     e->flags.reserve(4);
@@ -366,6 +544,10 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
     double* hh = e->hh.get();
     double* part0 = e->part.get();
     double* part1 = part0 + pilu::kVecBlocks;
+    double* tnorm = part0 + 2 * pilu::kVecBlocks;
+    int* mgs_err = e->flags.get() + 1;
+    PILU_HIP(hipMemsetAsync(mgs_err, 0, sizeof(int), s));
+    const uint64_t wd = pilu::watchdog_ticks(10.0);
     std::vector<TimedPair> timed;
     size_t ev_used = 0;
     auto next_event = [&]() {
@@ -399,6 +581,10 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
         // (its inner loop never runs); report failure instead.
         if (std::isnan(h.beta)) { ok = false; break; }
         pilu::k_scale<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(n, vv, dctl, false);
+        if (pbuf) {
+            const long cnt = static_cast<long>(2) * (m + 1) * mgs.nwg;
+            pilu::k_fill_pending<<<grid_for_n(cnt, 256), 256, 0, s>>>(cnt, reinterpret_cast<uint64_t*>(pbuf));
+        }
         for (int i = 0; i < m; ++i) {
             double* vi = vv + static_cast<long>(i) * n;
             double* vi1 = vv + static_cast<long>(i + 1) * n;
@@ -410,17 +596,23 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
             t_begin(0);
             spmv(lanes, s, n, rp, ci, val, 1.0, zi, 0.0, nullptr, vi1, vv, false, part0, step_gate);
             t_end();
-            double* pin = part0;
-            double* pout = part1;
-            for (int j = 0; j <= i; ++j) {
-                const double* vnext = j < i ? vv + static_cast<long>(j + 1) * n : nullptr;
-                pilu::k_mgs<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(
-                    n, vv + static_cast<long>(j) * n, vi1, vnext, pin, pout, hh + static_cast<long>(i) * im1 + j,
-                    step_gate);
-                std::swap(pin, pout);
+            if (pbuf) {
+                launch_mgs_all(mgs, s, n, i, m, vv, vi1, part0, pbuf, hh + static_cast<long>(i) * im1, tnorm, dctl,
+                               mgs_err, wd);
+                pilu::k_arnoldi_finish<<<1, pilu::kVecThreads, 0, s>>>(dctl, nullptr, tnorm, hh, e->hist.get());
+            } else {
+                double* pin = part0;
+                double* pout = part1;
+                for (int j = 0; j <= i; ++j) {
+                    const double* vnext = j < i ? vv + static_cast<long>(j + 1) * n : nullptr;
+                    pilu::k_mgs<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(
+                        n, vv + static_cast<long>(j) * n, vi1, vnext, pin, pout, hh + static_cast<long>(i) * im1 + j,
+                        step_gate);
+                    std::swap(pin, pout);
+                }
+                pilu::k_arnoldi_finish<<<1, pilu::kVecThreads, 0, s>>>(dctl, pin, nullptr, hh, e->hist.get());
+                pilu::k_scale<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(n, vi1, dctl, true);
             }
-            pilu::k_arnoldi_finish<<<1, pilu::kVecThreads, 0, s>>>(dctl, pin, hh, e->hist.get());
-            pilu::k_scale<<<pilu::kVecBlocks, pilu::kVecThreads, 0, s>>>(n, vi1, dctl, true);
             PILU_HIP(hipMemcpyAsync(&e->snap[i & 1], dctl, sizeof(h), hipMemcpyDeviceToHost, s));
             PILU_HIP(hipEventRecord(e->snap_ev[i & 1], s));
             if (i >= 1) {
@@ -444,6 +636,12 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
     pilu::k_sqrt_sum<<<1, pilu::kVecThreads, 0, s>>>(part0, part1);
     PILU_HIP(hipMemcpyAsync(&e->true_res, part1, sizeof(double), hipMemcpyDeviceToHost, s));
     pilu::check_sweep_errors(ie);
+    if (pbuf) {
+        int werr = 0;
+        PILU_HIP(hipMemcpyAsync(&werr, mgs_err, sizeof(int), hipMemcpyDeviceToHost, s));
+        PILU_HIP(hipStreamSynchronize(s));
+        if (werr & pilu::kErrTimeout) throw std::runtime_error("GMRES: Gram-Schmidt reduction wait timed out (watchdog)");
+    }
     st.total_fgmr_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     for (const auto& p : timed) {
         float ms = 0.f;

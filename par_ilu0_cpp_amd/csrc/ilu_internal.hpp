@@ -86,6 +86,7 @@ struct lljbash::IluSolver::Ext {
     // tiles of the grid (tile[] != 0, grid[] declared by SetGridShape).
     int blk_rows = 0, nblk = 0, max_block_rows = 0;
     int tile[3] = {0, 0, 0}, grid[3] = {0, 0, 0};
+    int skew[3] = {0, 0, 0};  // tile coordinates u = x + skew0 y + skew1 z, v = y + skew2 z
     std::vector<int> h_block_of;
     pilu::DevBuf<int> block_of;
     int part_version = 0;

@@ -290,24 +290,67 @@ void use_blocks(Ext* e, int blk) {
     install_partition(e, ceil_div(e->n, blk), true);
 }
 
-// tx x ty x tz tiles of the grid declared by SetGridShape (row
-// x + nx (y + ny z)).
+// Tiles of the grid declared by SetGridShape (row x + nx (y + ny z)).
+// Tiles are boxes of tx x ty x tz in skewed coordinates
+//   u = x + b y + c z,  v = y + c2 z,  w = z,
+// with the smallest non-negative integers b, c, c2 for which no forward
+// dependency (an entry of L, offset (dx, dy, dz) from its row) points to a
+// larger u, v or w: then the tile index never increases along a dependency
+// and the tiles form an acyclic graph.  A 7-point ILU(0) pattern gives
+// b = c = c2 = 0 (plain boxes); ILU(1), whose fill couples (x+1, y-1),
+// (x+1, z-1) and (y+1, z-1), gives b = c = c2 = 1.  The skew is unimodular,
+// so an interior tile holds exactly tx ty tz rows.
 void use_tiles(Ext* e, int tx, int ty, int tz) {
     const int nx = e->grid[0], ny = e->grid[1], nz = e->grid[2];
     if (nx <= 0) throw std::runtime_error("tiled sweeps need the grid shape (SetGridShape)");
     if (tx < 1 || ty < 1 || tz < 1 || static_cast<long>(tx) * ty * tz > kMaxBlockRows)
         throw std::runtime_error("tile must hold 1..16384 rows");
     if (e->tile[0] == tx && e->tile[1] == ty && e->tile[2] == tz) return;
-    const int ntx = ceil_div(nx, tx), nty = ceil_div(ny, ty), ntz = ceil_div(nz, tz);
-    e->h_block_of.resize(e->n);
-    for (int z = 0, i = 0; z < nz; ++z)
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x, ++i) e->h_block_of[i] = ((z / tz) * nty + y / ty) * ntx + x / tx;
+    const int n = e->n;
+    std::vector<int> rp(static_cast<size_t>(n) + 1), dg(n), ci(static_cast<size_t>(e->nnz));
+    PILU_HIP(hipMemcpy(rp.data(), e->rp.get(), sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+    PILU_HIP(hipMemcpy(dg.data(), e->dg.get(), sizeof(int) * dg.size(), hipMemcpyDeviceToHost));
+    PILU_HIP(hipMemcpy(ci.data(), e->ci.get(), sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+    const long nxy = static_cast<long>(nx) * ny;
+    auto ceil_div_signed = [](long a, long b) { return a >= 0 ? (a + b - 1) / b : -((-a) / b); };
+    long b = 0, c = 0, c2 = 0;
+    for (int pass = 0; pass < 2; ++pass) {  // b first, then c (which depends on b)
+        for (int r = 0; r < n; ++r) {
+            const long x = r % nx, y = (r / nx) % ny, z = r / nxy;
+            for (int p = rp[r]; p < dg[r]; ++p) {
+                const long q = ci[p];
+                const long dx = q % nx - x, dy = (q / nx) % ny - y, dz = q / nxy - z;
+                if (pass == 0) {
+                    if (dz == 0 && dy < 0) b = std::max(b, ceil_div_signed(dx, -dy));
+                    if (dz < 0) c2 = std::max(c2, ceil_div_signed(dy, -dz));
+                } else if (dz < 0) {
+                    c = std::max(c, ceil_div_signed(dx + b * dy, -dz));
+                }
+            }
+        }
+    }
+    const long umax = (nx - 1) + b * (ny - 1) + c * (nz - 1), vmax = (ny - 1) + c2 * (nz - 1);
+    const long ntu = umax / tx + 1, ntv = vmax / ty + 1, ntw = (nz - 1) / tz + 1;
+    if (ntu * ntv * ntw > (1L << 30)) throw std::runtime_error("tiled sweeps: too many tiles");
+    // dense ids of the non-empty tiles, in (w, v, u) order
+    std::vector<int> id(static_cast<size_t>(ntu * ntv * ntw), -1);
+    e->h_block_of.resize(n);
+    int nblk = 0;
+    for (int r = 0; r < n; ++r) {
+        const long x = r % nx, y = (r / nx) % ny, z = r / nxy;
+        const long u = x + b * y + c * z, v = y + c2 * z;
+        const long t = ((z / tz) * ntv + v / ty) * ntu + u / tx;
+        if (id[t] < 0) id[t] = nblk++;
+        e->h_block_of[r] = id[t];
+    }
     e->blk_rows = 0;
     e->tile[0] = tx;
     e->tile[1] = ty;
     e->tile[2] = tz;
-    install_partition(e, ntx * nty * ntz, false);
+    e->skew[0] = static_cast<int>(b);
+    e->skew[1] = static_cast<int>(c);
+    e->skew[2] = static_cast<int>(c2);
+    install_partition(e, nblk, false);
 }
 
 // Device pattern is in e->rp / e->ci: validate, locate diagonals, schedule.
@@ -1015,10 +1058,12 @@ std::string IluSolver::DescribeSchedule() const {
     std::snprintf(buf, sizeof(buf),
                   "{\"n\": %d, \"nnz\": %ld, \"max_row\": %d, \"precision\": %d, \"levels_l\": %d, "
                   "\"levels_u\": %d, \"grid\": [%d, %d, %d], \"blk_rows\": %d, \"tile\": [%d, %d, %d], "
+                  "\"skew\": [%d, %d, %d], "
                   "\"blocks\": %d, \"max_block_rows\": %d, \"mode_l\": %d, \"lanes_l\": %d, \"mode_u\": %d, "
                   "\"lanes_u\": %d, ",
                   e->n, e->nnz, e->max_row, e->precision, e->nlev_l, e->nlev_u, e->grid[0], e->grid[1], e->grid[2],
-                  e->blk_rows, e->tile[0], e->tile[1], e->tile[2], e->nblk, e->max_block_rows, e->mode_l,
+                  e->blk_rows, e->tile[0], e->tile[1], e->tile[2], e->skew[0], e->skew[1], e->skew[2], e->nblk,
+                  e->max_block_rows, e->mode_l,
                   pilu::effective_lanes(e, false), e->mode_u, pilu::effective_lanes(e, true));
     std::string js(buf);
     const bool live_l = e->nblk > 0 && e->pack_l.version == e->part_version;

@@ -280,7 +280,11 @@ def test_tiled_sweeps_bit_exact(oracle, tile, lof):
     ilu.SetGridShape(s, s, s)
     ilu.SetSweepTiles(2, 2, *tile)
     sch = ilu.Schedule()
-    assert sch["tile"] == list(tile) and sch["blocks"] == np.prod([-(-s // t) for t in tile])
+    # ILU(0) tiles are plain boxes; ILU(1)'s fill needs u = x+y+z, v = y+z
+    assert sch["tile"] == list(tile) and sch["skew"] == ([0, 0, 0] if lof == 0 else [1, 1, 1])
+    assert sch["max_block_rows"] <= np.prod(tile)
+    if lof == 0:
+        assert sch["blocks"] == np.prod([-(-s // t) for t in tile])
     rng = np.random.default_rng(sum(tile) + lof)
     for g in (1, 2, 8, 16):
         ilu.SetSweepLanes(g, g)
@@ -293,7 +297,7 @@ def test_tiled_sweeps_bit_exact(oracle, tile, lof):
     assert ilu.SweepConfig()["tile"] == list(tile)
 
 
-def test_tiles_need_a_grid_and_an_acyclic_partition():
+def test_tiles_need_a_grid_and_follow_the_dependencies():
     a = gen.convdiff3d(6)
     ilu = factored(a.row_ptr, a.col_idx, a.value)
     with pytest.raises(RuntimeError):
@@ -305,17 +309,17 @@ def test_tiles_need_a_grid_and_an_acyclic_partition():
         ilu.SetSweepTiles(1, 1, 2, 2, 2)  # the CSR block sweep needs contiguous blocks
     with pytest.raises(RuntimeError):
         ilu.SetSweepTiles(2, 2, 64, 64, 8)  # more than 16384 rows per tile
-    # A 4 x 2 grid in two 2 x 2 tiles, tile 0 = rows {0,1,4,5}, tile 1 =
-    # {2,3,6,7}: row 2 needs row 1 and row 4 needs row 3, a cycle between
-    # the tiles, so no block order exists.
+    # A 4 x 2 grid: row 2 needs row 1 and row 4 = (0, 1) needs row 3 =
+    # (3, 0).  Plain 2 x 2 boxes would depend on each other both ways; the
+    # offset (+3, -1) makes the tiles skew by u = x + 3y, which breaks the
+    # cycle.
     rp = np.array([0, 1, 2, 4, 5, 7, 8, 9, 10], dtype=np.int32)
     ci = np.array([0, 1, 1, 2, 3, 3, 4, 5, 6, 7], dtype=np.int32)
     va = np.where(ci == np.repeat(np.arange(8), np.diff(rp)), 4.0, -1.0)
     ilu2 = factored(rp, ci, va)
     ilu2.SetGridShape(4, 2, 1)
-    with pytest.raises(RuntimeError):
-        ilu2.SetSweepTiles(2, 2, 2, 2, 1)
-    ilu2.SetSweepTiles(2, 2, 4, 1, 1)  # row-major tiles are contiguous blocks: fine
+    ilu2.SetSweepTiles(2, 2, 2, 2, 1)
+    assert ilu2.Schedule()["skew"] == [3, 0, 0]
     b = np.arange(8.0)
     ilu2.SetSweepLanes(1, 1)
     x = ilu2.Substitute(b)
