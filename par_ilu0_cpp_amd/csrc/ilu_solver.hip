@@ -523,18 +523,21 @@ void build_passes(Ext* e, SweepPack& pk, int G) {
 }
 
 // Refresh the packed values from the current factors.
+// The value arrays always get at least one element and the diagonal is
+// always gathered: the packed sweep reads pval[0] from idle lanes and needs
+// the U diagonal even when a direction has no off-diagonal entry at all.
 void gather_pack(Ext* e, SweepPack& pk, bool upper) {
-    if (pk.ne == 0 || pk.version != e->part_version) return;
+    if (pk.version != e->part_version) return;
     hipStream_t s = e->stream;
     const int grid = 2048;
     if (e->precision == 32) {
-        pk.pval32.reserve(pk.ne);
+        pk.pval32.reserve(std::max<long>(pk.ne, 1));
         if (upper) pk.pdiag32.reserve(e->n);
         k_pack_gather<float, float><<<grid, 256, 0, s>>>(pk.ne, pk.psrc.get(), e->fac32.get(), pk.pval32.get(), e->n,
                                                          reinterpret_cast<const int2*>(pk.rowinfo.get()), e->dg.get(),
                                                          upper ? pk.pdiag32.get() : nullptr);
     } else {
-        pk.pval64.reserve(pk.ne);
+        pk.pval64.reserve(std::max<long>(pk.ne, 1));
         if (upper) pk.pdiag64.reserve(e->n);
         k_pack_gather<double, double><<<grid, 256, 0, s>>>(pk.ne, pk.psrc.get(), e->fac.get(), pk.pval64.get(), e->n,
                                                            reinterpret_cast<const int2*>(pk.rowinfo.get()),
@@ -614,6 +617,14 @@ void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double
     build_passes(e, pk, G);
     const size_t lds = pp_lds_bytes(e, pk);
     if (lds > kMaxLds) throw std::runtime_error("packed sweep: block and pass list exceed the LDS");
+    // the kernel reads these unconditionally (clamped indices): refuse to
+    // launch on missing arrays
+    const size_t nv = sizeof(T) == 4 ? pk.pval32.size() : pk.pval64.size();
+    const size_t nd = sizeof(T) == 4 ? pk.pdiag32.size() : pk.pdiag64.size();
+    if (!pval || nv < static_cast<size_t>(std::max<long>(pk.ne, 1)) || (UPPER && (!pdiag || nd < static_cast<size_t>(e->n))) ||
+        !pk.kptr.get() || !pk.desc_ptr.get() || !pk.pdesc.get() || pk.rowinfo.size() < 2 * static_cast<size_t>(e->n) ||
+        pk.pcol.size() < static_cast<size_t>(std::max<long>(pk.ne, 1)))
+        throw std::runtime_error("packed sweep: pack arrays missing (factorize before sweeping)");
     auto kern = k_sweep_pp<G, S, kPpUnroll, kPpUnroll / 2, T, UPPER>;
     PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
@@ -930,10 +941,9 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     for (int blk : {4096, 8192, 16384})
         if (blk <= e->n) parts.push_back({blk, 0, 0, 0});
     if (e->grid[0] > 0) {
-        static const int kTiles[5][3] = {{16, 16, 32}, {32, 16, 16}, {16, 16, 16}, {32, 32, 8}, {16, 16, 64}};
+        static const int kTiles[5][3] = {{32, 16, 16}, {24, 24, 24}, {16, 16, 32}, {16, 16, 16}, {32, 32, 16}};
         for (const auto& t : kTiles)
-            if (t[0] <= e->grid[0] && t[1] <= e->grid[1] && t[2] <= e->grid[2] &&
-                static_cast<long>(t[0]) * t[1] * t[2] < e->n)
+            if (t[2] <= e->grid[2] && static_cast<long>(t[0]) * t[1] * t[2] < e->n)
                 parts.push_back({0, t[0], t[1], t[2]});
     }
     pilu::DevBuf<double> b(e->n), x(e->n);
