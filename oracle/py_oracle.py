@@ -165,3 +165,44 @@ def gmres(a, rhs, tol, m, maxits, factors=None, x0=None, fast: bool = False):
     ok = lib.orc_gmres(n, _pi(arp), _pi(aci), _pd(ava), *fp, _pd(b), _pd(x), float(tol), int(m), int(maxits),
                        C.byref(its), _pd(hist), hist.size, C.byref(nh))
     return bool(ok == 1), its.value, x, hist[: nh.value].copy()
+
+
+_OMP = None
+
+
+def _load_omp() -> C.CDLL:
+    global _OMP
+    if _OMP is None:
+        lib = C.CDLL(str(HERE / "liboracle_omp.so"))
+        lib.baseline_threads.restype = C.c_int
+        lib.baseline_set_threads.argtypes = [C.c_int]
+        lib.baseline_ilu_gmres.restype = C.c_int
+        lib.baseline_ilu_gmres.argtypes = [C.c_int, _I, _I, _D, _I, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int,
+                                           C.POINTER(C.c_int), _D]
+        _OMP = lib
+    return _OMP
+
+
+def baseline_threads(k: int | None = None) -> int:
+    """Threads of the OpenMP baseline (set them first when k is given)."""
+    lib = _load_omp()
+    if k is not None:
+        lib.baseline_set_threads(int(k))
+    return int(lib.baseline_threads())
+
+
+def baseline_ilu_gmres(a, factors_pattern, rhs, tol, m, maxits):
+    """OpenMP level-scheduled ILU + GMRES(m) (baseline_omp.c).
+    factors_pattern = (frp, fci, fval before factorization).
+    -> (ret, its, x, times[schedule, factorize, gmres])."""
+    lib = _load_omp()
+    arp, aci, aval = _i(a[0]), _i(a[1]), _d(a[2])
+    frp, fci, fval = _i(factors_pattern[0]), _i(factors_pattern[1]), _d(factors_pattern[2]).copy()
+    dg = diag_ptr(frp, fci)
+    b = _d(rhs)
+    x = np.zeros_like(b)
+    its = C.c_int(0)
+    times = np.zeros(3)
+    ret = lib.baseline_ilu_gmres(arp.size - 1, _pi(arp), _pi(aci), _pd(aval), _pi(frp), _pi(fci), _pi(_i(dg)),
+                                 _pd(fval), _pd(b), _pd(x), float(tol), int(m), int(maxits), C.byref(its), _pd(times))
+    return ret, its.value, x, times

@@ -99,3 +99,23 @@ def test_solution_matches_direct_solve(oracle):
     ref = spla.spsolve(a.to_scipy().tocsc(), b)
     assert ok
     assert np.linalg.norm(x - ref) <= 1e-7 * np.linalg.norm(ref)
+
+
+def test_openmp_baseline_matches_serial_oracle(oracle):
+    """The threaded CPU baseline (level-scheduled ILU + GMRES) solves the
+    same systems as the serial oracle: same factors (identical per-row
+    arithmetic), iterations within one, true residual at the tolerance."""
+    from oracle import py_oracle as O
+
+    a = gen.convdiff3d(12)
+    b = gen.rhs_uniform(a.n, seed=3)
+    tol = 1e-8 * np.linalg.norm(b)
+    frp, fci, _, fval, _ = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
+    fac, dg, _ = oracle.ilu_factorize(frp, fci, fval)
+    ok_s, its_s, _, _ = oracle.gmres((a.row_ptr, a.col_idx, a.value), b, tol, 30, 500, factors=(frp, fci, dg, fac))
+    for threads in (1, 4):
+        O.baseline_threads(threads)
+        ret, its, x, times = O.baseline_ilu_gmres((a.row_ptr, a.col_idx, a.value), (frp, fci, fval), b, tol, 30, 500)
+        assert ret == 1 and ok_s and abs(its - its_s) <= 1
+        assert np.linalg.norm(b - a.matvec(x)) <= 1.01 * tol
+        assert np.all(times >= 0)
