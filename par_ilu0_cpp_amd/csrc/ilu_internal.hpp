@@ -92,6 +92,7 @@ struct lljbash::IluSolver::Ext {
     int part_version = 0;
     long factor_version = 0;
     pilu::DevBuf<int> bperm_l, bperm_u, blev_l, blev_u, loff_l, loff_u;
+    bool analysed = false;  // pattern checked and schedules built (SetupMatrix / SymbolicDevice completed)
     bool factored = false;
     bool y_armed = false;
 };

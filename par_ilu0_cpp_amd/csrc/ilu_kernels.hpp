@@ -74,49 +74,59 @@ __global__ void k_iota(int n, int* __restrict__ p) {
 // ---------------------------------------------------------------------
 // Level of each row in the L graph (row i needs rows of L(i)) or the U
 // graph (row i needs rows of U(i) \ {i}): level = 1 + max level of its
-// dependencies, 0 without any.  One lane per row; rows are visited in
-// natural order for L and reverse order for U, which is a topological
-// order, so a lane only waits on rows owned by earlier tickets or by its
-// own wave.  level[] must be -1 on entry.  maxlev[0] receives the deepest
-// level.  blk > 0 restricts the graph to dependencies inside the row's block
-// of blk consecutive rows (local levels of the block sweep).
+// dependencies, 0 without any.  G lanes per row, each reading every G-th
+// dependency (long rows would otherwise make every level a chain of
+// dependent reads); rows are visited in natural order for L and reverse
+// order for U, which is a topological order, so a row only waits on rows
+// owned by earlier tickets or by its own wave.  level[] must be -1 on
+// entry.  maxlev[0] receives the deepest level.  block_of (optional)
+// restricts the graph to dependencies inside the row's block (local levels
+// of the block-local sweeps).
 // ---------------------------------------------------------------------
-template <bool UPPER>
+template <bool UPPER, int G>
 __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict__ rp,
                                                    const int* __restrict__ ci,
                                                    const int* __restrict__ dg, int* level,
                                                    int* maxlev, int* ctr, int* err, uint64_t wd_limit,
                                                    const int* __restrict__ block_of) {
+    constexpr int R = 64 / G;              // rows per wave
+    constexpr int ROWS = (kBlock / 64) * R;  // rows per workgroup
     const int ticket = take_ticket(ctr);
     const Watchdog wd(wd_limit);
-    const int k = ticket * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int t = lane & (G - 1);
+    const long k = static_cast<long>(ticket) * ROWS + (threadIdx.x >> 6) * R + lane / G;
     const bool active = k < n;
-    const int row = active ? (UPPER ? n - 1 - k : k) : 0;
+    const int row = active ? static_cast<int>(UPPER ? n - 1 - k : k) : 0;
     const int my_block = block_of && active ? block_of[row] : 0;
     int e = 0, end = 0;
     if (active) {
-        e = UPPER ? dg[row] + 1 : rp[row];
+        e = (UPPER ? dg[row] + 1 : rp[row]) + t;
         end = UPPER ? rp[row + 1] : dg[row];
     }
     int lv = 0;
     bool done = !active;
     while (true) {
         if (!done) {
+            // lane t reads entries t, t+G, ...; a pending level stops it until the next round
             while (e < end) {
                 const int c = ci[e];
-                if (block_of && block_of[c] != my_block) { ++e; continue; }  // outside the row's block
+                if (block_of && block_of[c] != my_block) { e += G; continue; }  // outside the row's block
                 const int d = peek_int(level + c);
                 if (d < 0) break;
                 lv = d + 1 > lv ? d + 1 : lv;
-                ++e;
+                e += G;
             }
             if (e < end && wd.expired()) {
                 atomicOr(err, kErrTimeout);
                 e = end;
             }
-            if (e >= end) {
-                publish_int(level + row, lv);
-                atomicMax(maxlev, lv);
+            if (group_all<G>(e >= end)) {
+                const int l = group_max<G>(lv);
+                if (t == 0) {
+                    publish_int(level + row, l);
+                    atomicMax(maxlev, l);
+                }
                 done = true;
             }
         }
@@ -327,6 +337,7 @@ template <int S, typename T>
 struct PpB {
     T v[S];
     T dv;
+    double bv;  // right-hand side of the row (flag-synchronised form only)
     uint64_t xg[S];
 };
 
@@ -354,9 +365,11 @@ __device__ __forceinline__ void pp_issue_a(const PassDesc& d, int gid, int t, in
     }
 }
 
-template <int G, int S, typename T, bool UPPER>
+template <int G, int S, typename T, bool UPPER, bool FLAGS>
 __device__ __forceinline__ void pp_issue_b(int t, int idle, const PpA<G, S>& a, const T* __restrict__ pval,
-                                           const T* __restrict__ pdiag, const double* out, PpB<S, T>& b) {
+                                           const T* __restrict__ pdiag, const double* rhs, const double* out,
+                                           PpB<S, T>& b) {
+    if (FLAGS) b.bv = rhs[a.row];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
@@ -378,7 +391,7 @@ __device__ __forceinline__ void pp_issue_b(int t, int idle, const PpA<G, S>& a, 
 
 // kptr[t]: schedule position of the first row of the t-th block processed;
 // pass_ptr[t]: its first PassDesc (each list padded to a multiple of DA).
-template <int G, int S, int DA, int DB, typename T, bool UPPER>
+template <int G, int S, int DA, int DB, typename T, bool UPPER, bool FLAGS>
 __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, const int* __restrict__ pass_ptr,
                                                   const PassDesc* __restrict__ desc,
                                                   const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
@@ -395,7 +408,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
     const int p0 = pass_ptr[ticket], np = pass_ptr[ticket + 1] - p0;
     for (int i = threadIdx.x; i < nrows; i += 512) {
         const int row = rowinfo[kb0 + i].x;
-        xs[i] = b[row];
+        xs[i] = FLAGS ? __longlong_as_double(kPending) : b[row];
         if (reset) reset[row] = __longlong_as_double(kPending);
     }
     for (int i = threadIdx.x; i < np; i += 512) sd[i] = desc[p0 + i];
@@ -414,7 +427,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
 #pragma unroll
     for (int u = 0; u < DA; ++u) pp_issue_a<G, S>(sd[u < np ? u : np - 1], gid, t, per, rowinfo, pcol, ra[u]);
 #pragma unroll
-    for (int u = 0; u < DB; ++u) pp_issue_b<G, S, T, UPPER>(t, idle, ra[u], pval, pdiag, out, rb[u]);
+    for (int u = 0; u < DB; ++u) pp_issue_b<G, S, T, UPPER, FLAGS>(t, idle, ra[u], pval, pdiag, b, out, rb[u]);
     // np is a multiple of DA: the unrolled body below has no pass guard, so
     // the issue sequence (and the compiler's count of outstanding requests)
     // is the same on every path.
@@ -429,14 +442,25 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
             PpB<S, T>& r = rb[u % DB];
             if (a.act) {
                 const int slot = a.kk - kb0;
-                double acc = t == 0 ? xs[slot] : 0.0;
+                double acc = t == 0 ? (FLAGS ? r.bv : xs[slot]) : 0.0;
+                // x of an in-block row: LDS (a value-flag in the flag-synchronised form)
+                auto local_x = [&](int c) -> double {
+                    if (!FLAGS) return xs[-1 - c];
+                    uint64_t bits;
+                    while ((bits = __hip_atomic_load(reinterpret_cast<uint64_t*>(xs - 1 - c), __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) == kPending) {
+                        if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
+                        spin_pause();
+                    }
+                    return __longlong_as_double(bits);
+                };
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     if (t + s * G < a.cnt) {
                         const int c = a.c[s];
                         double x;
                         if (c < 0) {
-                            x = xs[-1 - c];
+                            x = local_x(c);
                         } else {
                             uint64_t bits = r.xg[s];
                             while (bits == kPending) {
@@ -454,7 +478,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
                     const int c = pcol[pos];
                     double x;
                     if (c < 0) {
-                        x = xs[-1 - c];
+                        x = local_x(c);
                     } else {
                         uint64_t bits;
                         while ((bits = peek_bits(out + c)) == kPending) {
@@ -468,17 +492,24 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
                 const double sum = group_sum<G>(acc);
                 if (t == 0) {
                     const double res = UPPER ? sum / static_cast<double>(r.dv) : sum;
-                    xs[slot] = res;
+                    if (FLAGS) {
+                        uint64_t rb_ = __double_as_longlong(res);
+                        if (res != res) rb_ = kQuietNan;
+                        __hip_atomic_store(reinterpret_cast<uint64_t*>(xs + slot), rb_, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    } else {
+                        xs[slot] = res;
+                    }
                     publish(out + a.row, res);
                 }
             }
-            if (a.bar) {
+            if (!FLAGS && a.bar) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
                 __builtin_amdgcn_s_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             }
             pp_issue_a<G, S>(dn, gid, t, per, rowinfo, pcol, ra[u]);
-            pp_issue_b<G, S, T, UPPER>(t, idle, ra[(u + DB) % DA], pval, pdiag, out, rb[u % DB]);
+            pp_issue_b<G, S, T, UPPER, FLAGS>(t, idle, ra[(u + DB) % DA], pval, pdiag, b, out, rb[u % DB]);
         }
     }
     if (timed_out) atomicOr(err, kErrTimeout);

@@ -73,6 +73,24 @@ int read_int(const int* d, hipStream_t s) {
     return h;
 }
 
+template <bool UPPER, int G>
+void levels_g(Ext* e, const int* block_of, int* level, int* maxlev, uint64_t wd) {
+    constexpr int rows = (kBlock / 64) * (64 / G);
+    k_levels<UPPER, G><<<grid_for(e->n, rows), kBlock, 0, e->stream>>>(e->n, e->rp.get(), e->ci.get(), e->dg.get(),
+                                                                        level, maxlev, e->ctr.get(),
+                                                                        e->err.get() + kErrLevels, wd, block_of);
+}
+
+// Level analysis of one direction: lanes per row from the average number
+// of dependencies per row.
+template <bool UPPER>
+void launch_levels(Ext* e, const int* block_of, int* level, int* maxlev, uint64_t wd) {
+    const double avg = e->n > 0 ? static_cast<double>(e->nnz - e->n) / (2.0 * e->n) : 0.0;
+    if (avg <= 8) levels_g<UPPER, 1>(e, block_of, level, maxlev, wd);
+    else if (avg <= 32) levels_g<UPPER, 8>(e, block_of, level, maxlev, wd);
+    else levels_g<UPPER, 64>(e, block_of, level, maxlev, wd);
+}
+
 // Levels of one dependency graph and the rows sorted by (level, row).
 int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
     const int n = e->n;
@@ -83,13 +101,9 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
     const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
     k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
     PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
-    const int grid = grid_for(n, kBlock);
-    if (upper)
-        k_levels<true><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
-                                               e->err.get() + kErrLevels, wd, nullptr);
-    else
-        k_levels<false><<<grid, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev, ctr,
-                                                e->err.get() + kErrLevels, wd, nullptr);
+    (void)ctr;
+    if (upper) launch_levels<true>(e, nullptr, level, maxlev, wd);
+    else launch_levels<false>(e, nullptr, level, maxlev, wd);
     PILU_HIP(hipGetLastError());
     const int nlev = read_int(maxlev, s) + 1;
     if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
@@ -132,12 +146,8 @@ void build_block_schedule(Ext* e, bool upper, const std::vector<int>& pos, DevBu
     PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
     const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
     const int* bo = e->block_of.get();
-    if (upper)
-        k_levels<true><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level, maxlev,
-                                                              e->ctr.get(), e->err.get() + kErrLevels, wd, bo);
-    else
-        k_levels<false><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level,
-                                                               maxlev, e->ctr.get(), e->err.get() + kErrLevels, wd, bo);
+    if (upper) launch_levels<true>(e, bo, level, maxlev, wd);
+    else launch_levels<false>(e, bo, level, maxlev, wd);
     PILU_HIP(hipGetLastError());
     const int nlev = read_int(maxlev, s) + 1;
     if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
@@ -356,6 +366,8 @@ void use_tiles(Ext* e, int tx, int ty, int tz) {
 // Device pattern is in e->rp / e->ci: validate, locate diagonals, schedule.
 void analyse(Ext* e, const int* host_rp) {
     const int n = e->n;
+    e->analysed = false;  // until every schedule below is built
+    e->factored = false;
     hipStream_t s = e->stream;
     e->dg.reserve(n);
     e->level.reserve(n);
@@ -400,6 +412,7 @@ void analyse(Ext* e, const int* host_rp) {
     e->tile[0] = e->tile[1] = e->tile[2] = 0;
     e->grid[0] = e->grid[1] = e->grid[2] = 0;
     ++e->part_version;
+    e->analysed = true;
 }
 
 // Packed schedule-ordered factors of one direction for the current
@@ -610,7 +623,7 @@ bool pp_fits(Ext* e, int G) {
     return pp_lds_bytes(e, e->pack_l) <= kMaxLds && pp_lds_bytes(e, e->pack_u) <= kMaxLds;
 }
 
-template <int G, int S, typename T, bool UPPER>
+template <int G, int S, typename T, bool UPPER, bool FLAGS>
 void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double* out, double* reset, const int* gate,
                   int* ctr) {
     SweepPack& pk = UPPER ? e->pack_u : e->pack_l;
@@ -625,7 +638,7 @@ void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double
         !pk.kptr.get() || !pk.desc_ptr.get() || !pk.pdesc.get() || pk.rowinfo.size() < 2 * static_cast<size_t>(e->n) ||
         pk.pcol.size() < static_cast<size_t>(std::max<long>(pk.ne, 1)))
         throw std::runtime_error("packed sweep: pack arrays missing (factorize before sweeping)");
-    auto kern = k_sweep_pp<G, S, kPpUnroll, kPpUnroll / 2, T, UPPER>;
+    auto kern = k_sweep_pp<G, S, kPpUnroll, kPpUnroll / 2, T, UPPER, FLAGS>;
     PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
     kern<<<e->nblk, 512, lds, e->stream>>>(pk.kptr.get(), pk.desc_ptr.get(),
@@ -648,9 +661,15 @@ void sweep_pp_s(Ext* e, const double* b, double* out, double* reset, const int* 
         pdiag = UPPER ? pk.pdiag64.get() : nullptr;
     }
     const int need = (pk.max_w + G - 1) / G;
-    if (need <= 1) sweep_pp_one<G, 1, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
-    else if (need <= 2) sweep_pp_one<G, 2, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
-    else sweep_pp_one<G, 4, T, UPPER>(e, pval, pdiag, b, out, reset, gate, ctr);
+    if ((UPPER ? e->mode_u : e->mode_l) == 3) {
+        if (need <= 1) sweep_pp_one<G, 1, T, UPPER, true>(e, pval, pdiag, b, out, reset, gate, ctr);
+        else if (need <= 2) sweep_pp_one<G, 2, T, UPPER, true>(e, pval, pdiag, b, out, reset, gate, ctr);
+        else sweep_pp_one<G, 4, T, UPPER, true>(e, pval, pdiag, b, out, reset, gate, ctr);
+        return;
+    }
+    if (need <= 1) sweep_pp_one<G, 1, T, UPPER, false>(e, pval, pdiag, b, out, reset, gate, ctr);
+    else if (need <= 2) sweep_pp_one<G, 2, T, UPPER, false>(e, pval, pdiag, b, out, reset, gate, ctr);
+    else sweep_pp_one<G, 4, T, UPPER, false>(e, pval, pdiag, b, out, reset, gate, ctr);
 }
 
 template <typename T, bool UPPER>
@@ -668,7 +687,7 @@ template <typename T, bool UPPER>
 void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
                     const int* gate, int* ctr) {
     const int mode = UPPER ? e->mode_u : e->mode_l;
-    if (mode == 2 && e->nblk > 0) {
+    if (mode >= 2 && e->nblk > 0) {
         ensure_packs(e);
         sweep_pp_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
         return;
@@ -732,7 +751,8 @@ bool run_factor(Ext* e) {
 }
 
 Ext* need_setup(Ext* e) {
-    if (!e || e->n <= 0) throw std::runtime_error("IluSolver: SetupMatrix has not been called");
+    if (!e || e->n <= 0 || !e->analysed)
+        throw std::runtime_error("IluSolver: SetupMatrix has not been called or did not complete");
     return e;
 }
 
@@ -784,6 +804,7 @@ void IluSolver::SetupMatrix() {
         throw std::runtime_error("IluSolver: row_ptr inconsistent with the matrix storage");
     if (!ext_) ext_ = new Ext;
     Ext* e = ext_;
+    e->analysed = false;
     e->n = n;
     e->nnz = nnz;
     e->rp.reserve(static_cast<size_t>(n) + 1);
@@ -800,6 +821,7 @@ void IluSolver::SetupMatrixDevice(int n, int nnz, const int* d_row_ptr, const in
     if (n <= 0) throw std::runtime_error("IluSolver: empty matrix");
     if (!ext_) ext_ = new Ext;
     Ext* e = ext_;
+    e->analysed = false;
     e->n = n;
     e->nnz = nnz;
     e->rp.reserve(static_cast<size_t>(n) + 1);
@@ -837,6 +859,7 @@ bool IluSolver::FactorizeDevice(const double* d_values) {
 int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx, int lof) {
     if (!ext_) ext_ = new Ext;
     Ext* e = ext_;
+    e->analysed = false;  // the previous pattern is gone whether or not this completes
     pilu::SymbolicResult res;
     const int nnz_f = pilu::symbolic_device(n, d_row_ptr, d_col_idx, nullptr, lof, res, e->stream);
     if (nnz_f < 0) return -1;
@@ -990,14 +1013,16 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
                 if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 1; }
             }
         }
-        e->mode_l = e->mode_u = 2;
-        for (int g : kPackLanes) {
-            if (!pilu::pp_fits(e, g)) continue;
-            double tl, tu;
-            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
-            record(2, pt, g, tl, tu);
-            if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 2; }
-            if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 2; }
+        for (int mode : {2, 3}) {
+            e->mode_l = e->mode_u = mode;
+            for (int g : kPackLanes) {
+                if (!pilu::pp_fits(e, g)) continue;
+                double tl, tu;
+                pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+                record(mode, pt, g, tl, tu);
+                if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = mode; }
+                if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = mode; }
+            }
         }
         const double tot = std::min(bb[0], lev_best[0]) + std::min(bb[1], lev_best[1]);
         if (tot < best_total) {
@@ -1020,18 +1045,18 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     e->mode_u = pick_mode[1];
     e->lanes_l = pick_lanes[0];
     e->lanes_u = pick_lanes[1];
-    if (pick_mode[0] == 2 || pick_mode[1] == 2) pilu::ensure_packs(e);
+    if (pick_mode[0] >= 2 || pick_mode[1] >= 2) pilu::ensure_packs(e);
     pilu::show_algorithm(this);
     return rows;
 }
 
 void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
     Ext* e = pilu::need_setup(ext_);
-    if (mode_l < 0 || mode_l > 2 || mode_u < 0 || mode_u > 2) throw std::runtime_error("sweep mode is 0, 1 or 2");
+    if (mode_l < 0 || mode_l > 3 || mode_u < 0 || mode_u > 3) throw std::runtime_error("sweep mode is 0, 1, 2 or 3");
     if (mode_l > 0 || mode_u > 0) pilu::use_blocks(e, blk_rows);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
-    if (mode_l == 2 || mode_u == 2) pilu::ensure_packs(e);
+    if (mode_l >= 2 || mode_u >= 2) pilu::ensure_packs(e);
     pilu::show_algorithm(this);
 }
 
@@ -1046,12 +1071,12 @@ void IluSolver::SetGridShape(int nx, int ny, int nz) {
 
 void IluSolver::SetSweepTiles(int mode_l, int mode_u, int tx, int ty, int tz) {
     Ext* e = pilu::need_setup(ext_);
-    if (mode_l == 1 || mode_u == 1 || mode_l < 0 || mode_l > 2 || mode_u < 0 || mode_u > 2)
-        throw std::runtime_error("tiled sweeps use mode 0 or 2");
+    if (mode_l == 1 || mode_u == 1 || mode_l < 0 || mode_l > 3 || mode_u < 0 || mode_u > 3)
+        throw std::runtime_error("tiled sweeps use mode 0, 2 or 3");
     pilu::use_tiles(e, tx, ty, tz);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
-    if (mode_l == 2 || mode_u == 2) pilu::ensure_packs(e);
+    if (mode_l >= 2 || mode_u >= 2) pilu::ensure_packs(e);
     pilu::show_algorithm(this);
 }
 

@@ -90,7 +90,7 @@ def test_sweeps_all_lane_shapes(oracle, bits):
 
 
 @pytest.mark.parametrize("bits", [64, 32])
-@pytest.mark.parametrize("mode,blk", [(0, 0), (1, 512), (2, 512), (2, 4096)])
+@pytest.mark.parametrize("mode,blk", [(0, 0), (1, 512), (2, 512), (2, 4096), (3, 512), (3, 4096)])
 def test_sweeps_bit_exact_in_lane_order(oracle, bits, mode, blk):
     """Every sweep kernel, at every lanes-per-row shape, reproduces the
     oracle's lane-order sweep bit for bit (FP32 factors: the oracle gets the
@@ -191,7 +191,7 @@ def test_autotune_keeps_results(oracle):
     assert np.max(np.abs(x1 - x0)) <= 1e-12 * np.max(np.abs(x0))
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("blk", [64, 1000, 4096, 100000])
 @pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_block_sweep_bit_identical_to_level_sweep(oracle, mode, blk, lanes):
@@ -238,7 +238,7 @@ def test_block_sweep_random_pattern(oracle):
     b = np.random.default_rng(2).uniform(-1, 1, 5000)
     xo = oracle.substitute(rp, ci, dg, ofac, b)
     ilu.SetSweepLanes(4, 4)
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         ilu.SetSweepMode(mode, mode, 512)
         x = ilu.Substitute(b)
         assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
@@ -293,6 +293,8 @@ def test_tiled_sweeps_bit_exact(oracle, tile, lof):
         assert np.array_equal(ilu.Substitute(b), xo), g
         assert np.array_equal(ilu.Substitute(b), xo), g  # re-armed buffers
     ilu.SetSweepTiles(0, 2, *tile)
+    assert np.array_equal(ilu.Substitute(b), xo)
+    ilu.SetSweepTiles(3, 3, *tile)  # barrier-free form: LDS value-flags
     assert np.array_equal(ilu.Substitute(b), xo)
     assert ilu.SweepConfig()["tile"] == list(tile)
 
