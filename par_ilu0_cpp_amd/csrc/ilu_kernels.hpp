@@ -406,10 +406,17 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
     const int kb0 = kptr[ticket], nrows = kptr[ticket + 1] - kb0;
     PassDesc* sd = reinterpret_cast<PassDesc*>(xs + max_rows);
     const int p0 = pass_ptr[ticket], np = pass_ptr[ticket + 1] - p0;
+    // b may alias the companion buffer (in-place apply): the barrier form
+    // reads each b_i into LDS before re-arming it; the flag form reads b_i
+    // through the prefetch ring and re-arms row i once it has used it.
     for (int i = threadIdx.x; i < nrows; i += 512) {
         const int row = rowinfo[kb0 + i].x;
-        xs[i] = FLAGS ? __longlong_as_double(kPending) : b[row];
-        if (reset) reset[row] = __longlong_as_double(kPending);
+        if (FLAGS) {
+            xs[i] = __longlong_as_double(kPending);
+        } else {
+            xs[i] = b[row];
+            if (reset) reset[row] = __longlong_as_double(kPending);
+        }
     }
     for (int i = threadIdx.x; i < np; i += 512) sd[i] = desc[p0 + i];
     __syncthreads();
@@ -443,6 +450,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
             if (a.act) {
                 const int slot = a.kk - kb0;
                 double acc = t == 0 ? (FLAGS ? r.bv : xs[slot]) : 0.0;
+                if (FLAGS && t == 0 && reset) reset[a.row] = __longlong_as_double(kPending);
                 // x of an in-block row: LDS (a value-flag in the flag-synchronised form)
                 auto local_x = [&](int c) -> double {
                     if (!FLAGS) return xs[-1 - c];
