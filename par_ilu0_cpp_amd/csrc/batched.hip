@@ -111,16 +111,34 @@ __device__ __forceinline__ double system_sum(const double* part, const int* sys_
     return t;
 }
 
+// True when some instance of system s still needs this chunk: with
+// step == true, an instance running the current Arnoldi step; otherwise an
+// instance not yet finished.  Uniform over the block, so a chunk of a system
+// whose right-hand sides have all converged costs one small read.
+template <int K>
+__device__ __forceinline__ bool system_live(const BCtl* ctl, int s, bool step) {
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const BCtl& q = ctl[s * K + r];
+        any = any || (step ? q.step_active != 0 : q.done == 0);
+    }
+    return any;
+}
+
 // Y = alpha A X + beta Yin on one chunk per block; part[c*K+r] = partial of
-// dotv.Y (dotv != null) or Y.Y (nrm).
+// dotv.Y (dotv != null) or Y.Y (nrm).  With ctl, chunks of systems that
+// need no work (system_live) are skipped.
 template <int K>
 __global__ __launch_bounds__(kBThreads) void k_bspmv(const int* __restrict__ chunk_beg, const int* __restrict__ rp,
                                                      const int* __restrict__ ci, const double* __restrict__ val,
                                                      double alpha, const double* __restrict__ x, double beta,
                                                      const double* yin, double* y, const double* __restrict__ dotv,
-                                                     bool nrm, double* part) {
+                                                     bool nrm, double* part, const int* __restrict__ chunk_sys,
+                                                     const BCtl* ctl, bool step) {
     __shared__ double s_red[(kBThreads / 64) * K];
     const int c = blockIdx.x;
+    if (ctl && !system_live<K>(ctl, chunk_sys[c], step)) return;
     const int r = threadIdx.x % K;
     const int rb = chunk_beg[c], re = chunk_beg[c + 1];
     double acc = 0.0;
@@ -145,10 +163,11 @@ __global__ __launch_bounds__(kBThreads) void k_bmgs(const int* __restrict__ chun
                                                     const int* __restrict__ sys_chunk, const double* __restrict__ vj,
                                                     double* w, const double* __restrict__ vnext,
                                                     const double* part_in, double* part_out, double* hh, long hstride,
-                                                    int hpos) {
+                                                    int hpos, const BCtl* ctl) {
     __shared__ double s_red[(kBThreads / 64) * K];
     const int c = blockIdx.x;
     const int s = chunk_sys[c];
+    if (!system_live<K>(ctl, s, true)) return;
     const int r = threadIdx.x % K;
     const double h = system_sum(part_in, sys_chunk, s, r, K);
     if (c == sys_chunk[s] && threadIdx.x < K) hh[(static_cast<long>(s) * K + r) * hstride + hpos] = h;
@@ -490,13 +509,16 @@ private:
         graph_exec_ = nullptr;
     }
 
+    // gate: 0 every chunk, 1 chunks of unfinished systems, 2 chunks of systems running the current step
     void spmv_launch(const CsrMatrix* a, double alpha, const double* x, double beta, const double* yin, double* y,
-                     const double* dotv, bool nrm, double* part) {
+                     const double* dotv, bool nrm, double* part, int gate = 0) {
+        const BCtl* ctl = gate ? ctl_.get() : nullptr;
         switch (K_) {
 #define PILU_CASE(KK)                                                                                                 \
     case KK:                                                                                                          \
         k_bspmv<KK><<<nchunks_, kBThreads, 0, stream_>>>(chunk_beg_.get(), a->row_ptr, a->col_idx, a->value, alpha, \
-                                                          x, beta, yin, y, dotv, nrm, part);                          \
+                                                          x, beta, yin, y, dotv, nrm, part, chunk_sys_.get(), ctl,    \
+                                                          gate == 2);                                                 \
         break;
             PILU_CASE(1) PILU_CASE(2) PILU_CASE(4) PILU_CASE(8)
 #undef PILU_CASE
@@ -549,7 +571,7 @@ private:
 #define PILU_CASE(KK)                                                                                                \
     case KK:                                                                                                         \
         k_bmgs<KK><<<nchunks_, kBThreads, 0, stream_>>>(chunk_beg_.get(), chunk_sys_.get(), sys_chunk_.get(), vj, w, \
-                                                         vnext, pin, pout, hh_.get(), hstride, hpos);                \
+                                                         vnext, pin, pout, hh_.get(), hstride, hpos, ctl_.get());    \
         break;
             PILU_CASE(1) PILU_CASE(2) PILU_CASE(4) PILU_CASE(8)
 #undef PILU_CASE
@@ -586,7 +608,7 @@ private:
         double* p1 = p0 + static_cast<long>(nchunks_) * K_;
         double* v = v_.get();
         double* z = z_.get();
-        spmv_launch(a, -1.0, sol, 1.0, rhs, v, nullptr, true, p0);
+        spmv_launch(a, -1.0, sol, 1.0, rhs, v, nullptr, true, p0, 1);
         k_bcycle_begin<<<ib, 256, 0, stream_>>>(ni, K_, ctl_.get(), p0, sys_chunk_.get(), hh_.get(), hstride, m);
         scale_launch(v, 0);
         for (int i = 0; i < m; ++i) {
@@ -595,7 +617,7 @@ private:
             double* zi = z + i * nk;
             k_bgate<<<ib, 256, 0, stream_>>>(ni, ctl_.get(), i);
             sweeps(ie, vi, zi);
-            spmv_launch(a, 1.0, zi, 0.0, nullptr, vi1, v, false, p0);
+            spmv_launch(a, 1.0, zi, 0.0, nullptr, vi1, v, false, p0, 2);
             double* pin = p0;
             double* pout = p1;
             for (int j = 0; j <= i; ++j) {
