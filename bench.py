@@ -327,7 +327,7 @@ def cpu_baseline(a, b, tol, gpu_its, steps):
         "kind": "port",
         "sample": f"oracle OpenMP build (level-scheduled ILU(1) factorization and sweeps, threaded SpMV/BLAS-1, "
                   f"-O3 -ffast-math), {threads} threads: factorization + {best['steps']} GMRES(50) steps timed; "
-                  f"value = factorize + per-step x {gpu_its} GPU iterations",
+                  f"value = factorize + per-step x {gpu_its} steps (plain ILU(1)-GMRES(50) on the GPU)",
         "by_threads": {str(k): v for k, v in runs.items()},
         "symbolic_s": t_sym,
     }
