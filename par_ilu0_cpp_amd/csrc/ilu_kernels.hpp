@@ -316,7 +316,7 @@ __global__ __launch_bounds__(512) void k_sweep_blk(int n, int blk, int nblk, con
 
 // Passes per unrolled step of k_sweep_pp; every block's descriptor list is
 // padded to a multiple of it with empty passes (cb = 0: no rows).
-constexpr int kPpUnroll = 8;
+constexpr int kPpUnroll = 16;
 
 struct PassDesc {
     int k_lo;   // schedule position of the pass's first row
@@ -325,35 +325,34 @@ struct PassDesc {
     int jw;     // bits 0..15: index of the pass's first row within its group; bits 16..31: group width
 };
 
-template <int G, int S>
+__device__ __forceinline__ int pd_count(const PassDesc& d) { return d.cb & 0x7fffffff; }
+__device__ __forceinline__ int pd_j0(const PassDesc& d) { return d.jw & 0xffff; }
+__device__ __forceinline__ int pd_width(const PassDesc& d) { return d.jw >> 16; }
+
+// Per-lane data of one pass, requested DA passes ahead.
+template <int S>
 struct PpA {
-    int row, cnt, jj, kk;  // row, its entry count, slab column, schedule position
-    int base, count, width;  // the pass's level group
-    bool act, bar;
+    int row, cnt, kk;  // row, its entry count, schedule position (clamped into the pass)
+    bool act;
     int c[S];
 };
 
+// Values of one pass, requested DB passes ahead.
 template <int S, typename T>
 struct PpB {
     T v[S];
     T dv;
     double bv;  // right-hand side of the row (flag-synchronised form only)
-    uint64_t xg[S];
 };
 
 template <int G, int S>
 __device__ __forceinline__ void pp_issue_a(const PassDesc& d, int gid, int t, int per,
                                            const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
-                                           PpA<G, S>& a) {
-    a.count = d.cb & 0x7fffffff;
-    a.bar = d.cb < 0;
-    a.base = d.base;
-    const int j0 = d.jw & 0xffff;
-    a.width = d.jw >> 16;
-    const int nact = a.count - j0 < per ? a.count - j0 : per;
+                                           PpA<S>& a) {
+    const int count = pd_count(d), j0 = pd_j0(d), width = pd_width(d);
+    const int nact = count - j0 < per ? count - j0 : per;
     a.act = gid < nact;
     const int off = a.act ? gid : 0;
-    a.jj = j0 + off;
     a.kk = d.k_lo + off;
     const int2 ri = rowinfo[a.kk];
     a.row = ri.x;
@@ -361,21 +360,27 @@ __device__ __forceinline__ void pp_issue_a(const PassDesc& d, int gid, int t, in
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
-        a.c[s] = pcol[q < a.width ? a.base + q * a.count + a.jj : 0];
+        a.c[s] = pcol[q < width ? d.base + q * count + j0 + off : 0];
     }
 }
 
 template <int G, int S, typename T, bool UPPER, bool FLAGS>
-__device__ __forceinline__ void pp_issue_b(int t, int idle, const PpA<G, S>& a, const T* __restrict__ pval,
-                                           const T* __restrict__ pdiag, const double* rhs, const double* out,
-                                           PpB<S, T>& b) {
-    if (FLAGS) b.bv = rhs[a.row];
+__device__ __forceinline__ void pp_issue_b(const PassDesc& d, int t, const PpA<S>& a, const T* __restrict__ pval,
+                                           const T* __restrict__ pdiag, const double* rhs, PpB<S, T>& b) {
+    const int count = pd_count(d), width = pd_width(d);
+    const int jj = pd_j0(d) + (a.kk - d.k_lo);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
-        b.v[s] = pval[q < a.width ? a.base + q * a.count + a.jj : 0];
+        b.v[s] = pval[q < width ? d.base + q * count + jj : 0];
     }
     if (UPPER) b.dv = pdiag[a.kk];
+    if (FLAGS) b.bv = rhs[a.row];
+}
+
+// Early reads of the out-of-block x of one pass, requested DX passes ahead.
+template <int G, int S>
+__device__ __forceinline__ void pp_issue_x(int t, int idle, const PpA<S>& a, const double* out, uint64_t* xg) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int q = t + s * G;
@@ -385,20 +390,24 @@ __device__ __forceinline__ void pp_issue_b(int t, int idle, const PpA<G, S>& a, 
         const bool need = a.act && q < a.cnt && c >= 0;
         int idx = need ? c : idle;
         asm volatile("" : "+v"(idx));  // keep the read unconditional (a select, not a branch)
-        b.xg[s] = peek_bits(out + idx);
+        xg[s] = peek_bits(out + idx);
     }
 }
 
 // kptr[t]: schedule position of the first row of the t-th block processed;
 // pass_ptr[t]: its first PassDesc (each list padded to a multiple of DA).
-template <int G, int S, int DA, int DB, typename T, bool UPPER, bool FLAGS>
+// Per pass t the requests are issued in the order they are consumed: the
+// early x reads of pass t+DX, the values of pass t+DB, the row data of pass
+// t+DA (DX < DB < DA).  The first two only depend on row data requested
+// DA-DB or more passes earlier, so no issue waits on a young request.
+template <int G, int S, int DA, int DB, int DX, typename T, bool UPPER, bool FLAGS>
 __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, const int* __restrict__ pass_ptr,
                                                   const PassDesc* __restrict__ desc,
                                                   const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
                                                   const T* __restrict__ pval, const T* __restrict__ pdiag,
                                                   const double* b, double* out, double* reset, const int* gate,
                                                   int* ctr, int* err, uint64_t wd_limit, int max_rows) {
-    static_assert(DA % DB == 0 && DA > DB && DA == kPpUnroll, "ring sizes");
+    static_assert(DA % DB == 0 && DA % DX == 0 && DX < DB && DB < DA && DA == kPpUnroll, "ring sizes");
     extern __shared__ double xs[];
     if (gate && *gate == 0) return;
     const int ticket = take_ticket(ctr);
@@ -429,12 +438,16 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
     const int gid = threadIdx.x / G;
     const int t = threadIdx.x & (G - 1);
     bool timed_out = false;
-    PpA<G, S> ra[DA];
+    PpA<S> ra[DA];
     PpB<S, T> rb[DB];
+    uint64_t rx[DX][S];
+    auto pass_desc = [&](int q) -> PassDesc { return sd[q < np ? q : np - 1]; };
 #pragma unroll
-    for (int u = 0; u < DA; ++u) pp_issue_a<G, S>(sd[u < np ? u : np - 1], gid, t, per, rowinfo, pcol, ra[u]);
+    for (int u = 0; u < DA; ++u) pp_issue_a<G, S>(pass_desc(u), gid, t, per, rowinfo, pcol, ra[u]);
 #pragma unroll
-    for (int u = 0; u < DB; ++u) pp_issue_b<G, S, T, UPPER, FLAGS>(t, idle, ra[u], pval, pdiag, b, out, rb[u]);
+    for (int u = 0; u < DX; ++u) pp_issue_x<G, S>(t, idle, ra[u], out, rx[u]);
+#pragma unroll
+    for (int u = 0; u < DB; ++u) pp_issue_b<G, S, T, UPPER, FLAGS>(pass_desc(u), t, ra[u], pval, pdiag, b, rb[u]);
     // np is a multiple of DA: the unrolled body below has no pass guard, so
     // the issue sequence (and the compiler's count of outstanding requests)
     // is the same on every path.
@@ -442,11 +455,14 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
 #pragma unroll
         for (int u = 0; u < DA; ++u) {
             const int q = p + u;
-            // descriptor of the pass issued at the end of this one, read now
-            // so that the LDS latency overlaps the row work
-            const PassDesc dn = sd[q + DA < np ? q + DA : np - 1];
-            PpA<G, S>& a = ra[u];
+            // descriptors used by this pass and its issues, read now so that
+            // the LDS latency overlaps the row work
+            const PassDesc d = sd[q];
+            const PassDesc db = pass_desc(q + DB);
+            const PassDesc da = pass_desc(q + DA);
+            PpA<S>& a = ra[u];
             PpB<S, T>& r = rb[u % DB];
+            uint64_t* xg = rx[u % DX];
             if (a.act) {
                 const int slot = a.kk - kb0;
                 double acc = t == 0 ? (FLAGS ? r.bv : xs[slot]) : 0.0;
@@ -470,7 +486,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
                         if (c < 0) {
                             x = local_x(c);
                         } else {
-                            uint64_t bits = r.xg[s];
+                            uint64_t bits = xg[s];
                             while (bits == kPending) {
                                 if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
                                 spin_pause();
@@ -481,8 +497,10 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
                         acc = fma(-x, static_cast<double>(r.v[s]), acc);
                     }
                 }
+                const int count = pd_count(d);
+                const int jj = pd_j0(d) + (a.kk - d.k_lo);
                 for (int qq = t + S * G; qq < a.cnt; qq += G) {
-                    const int pos = a.base + qq * a.count + a.jj;
+                    const int pos = d.base + qq * count + jj;
                     const int c = pcol[pos];
                     double x;
                     if (c < 0) {
@@ -511,13 +529,16 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
                     publish(out + a.row, res);
                 }
             }
-            if (!FLAGS && a.bar) {
+            if (!FLAGS && d.cb < 0) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
                 __builtin_amdgcn_s_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             }
-            pp_issue_a<G, S>(dn, gid, t, per, rowinfo, pcol, ra[u]);
-            pp_issue_b<G, S, T, UPPER, FLAGS>(t, idle, ra[(u + DB) % DA], pval, pdiag, b, out, rb[u % DB]);
+            // requests in consumption order: x of pass q+DX, values of pass
+            // q+DB, row data of pass q+DA (into the slot pass q just freed)
+            pp_issue_x<G, S>(t, idle, ra[(u + DX) % DA], out, rx[u % DX]);
+            pp_issue_b<G, S, T, UPPER, FLAGS>(db, t, ra[(u + DB) % DA], pval, pdiag, b, rb[u % DB]);
+            pp_issue_a<G, S>(da, gid, t, per, rowinfo, pcol, ra[u]);
         }
     }
     if (timed_out) atomicOr(err, kErrTimeout);

@@ -638,7 +638,7 @@ void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double
         !pk.kptr.get() || !pk.desc_ptr.get() || !pk.pdesc.get() || pk.rowinfo.size() < 2 * static_cast<size_t>(e->n) ||
         pk.pcol.size() < static_cast<size_t>(std::max<long>(pk.ne, 1)))
         throw std::runtime_error("packed sweep: pack arrays missing (factorize before sweeping)");
-    auto kern = k_sweep_pp<G, S, kPpUnroll, kPpUnroll / 2, T, UPPER, FLAGS>;
+    auto kern = k_sweep_pp<G, S, kPpUnroll, 4, 2, T, UPPER, FLAGS>;
     PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
     kern<<<e->nblk, 512, lds, e->stream>>>(pk.kptr.get(), pk.desc_ptr.get(),
