@@ -55,10 +55,12 @@ def parse():
     p.add_argument("--forms", default="fp64,fp32,fp32-learned")
     p.add_argument("--cpu-steps", type=int, default=50,
                    help="Arnoldi steps in the bounded CPU baseline sample (0 = skip)")
-    p.add_argument("--no-autotune", action="store_true", help="keep the default sweep shape")
-    p.add_argument("--sweep", default="",
-                   help="fixed sweep shape MODE_L:LANES_L:MODE_U:LANES_U:BLOCK instead of autotuning, "
-                        "BLOCK = rows of contiguous blocks or a tile TXxTYxTZ")
+    p.add_argument("--sweep", default="3:8:3:8:4096",
+                   help="sweep shape MODE_L:LANES_L:MODE_U:LANES_U:BLOCK, BLOCK = rows of contiguous blocks or a "
+                        "tile TXxTYxTZ; 'auto' times every shape after the first factorization (untimed setup); "
+                        "'default' keeps the library's level-scheduled default.  The default is the shape the "
+                        "autotuner picks on the 128^3 workload (pipelined packed sweeps with LDS value-flags, "
+                        "8 lanes per row, 4096-row blocks)")
     p.add_argument("--traffic", default=str(ROOT / "profiles" / "sweep_traffic_convdiff128.txt"),
                    help="rocprofv3 summary holding the measured HBM bytes per apply")
     p.add_argument("--copy-calib", action="store_true",
@@ -153,7 +155,7 @@ def main():
         log(f"[bench] {form}: symbolic+analysis {t_setup*1e3:.1f} ms nnz(F)={nnz_f} levels L={nl} U={nu}")
         sweep_cfg = ilu.SweepConfig()
         ilu.SetGridShape(s, s, s)  # the workload's grid: lets the sweeps use 3D tiles
-        if args.sweep:
+        if args.sweep not in ("auto", "default"):
             ml, ll, mu, lu, blk = args.sweep.split(":")
             ilu.SetSweepLanes(int(ll), int(lu))
             if "x" in blk:
@@ -162,7 +164,7 @@ def main():
                 ilu.SetSweepMode(int(ml), int(mu), int(blk))
             sweep_cfg = ilu.SweepConfig()
             log(f"[bench] {form}: fixed sweep shape {sweep_cfg}")
-        elif not args.no_autotune:
+        elif args.sweep == "auto":
             # one factorization, then time every sweep shape (part of setup, untimed)
             if ilu.FactorizeDevice(d_va.data_ptr(), True) != 0:
                 raise SystemExit("factorization failed")
