@@ -244,6 +244,28 @@ def test_block_sweep_random_pattern(oracle):
         assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
 
 
+def test_packed_sweep_long_rows(oracle):
+    """Rows longer than the packed sweep's register slots (at most 4 per
+    lane) finish in the kernel's tail loop: rows of ~150 entries per
+    direction, bit-identical to the oracle's lane-order sweep and to the
+    level sweep with the same lanes, in both packed forms."""
+    rp, ci = random_pattern(1200, 150, 400, seed=7)
+    va = dominant_values(rp, ci, seed=7)
+    ilu = factored(rp, ci, va)
+    ofac, dg, _ = oracle.ilu_factorize(rp, ci, va)
+    b = np.random.default_rng(3).uniform(-1, 1, rp.size - 1)
+    for g in (1, 4, 16):
+        ilu.SetSweepLanes(g, g)
+        ilu.SetSweepMode(0, 0, 0)
+        x_level = ilu.Substitute(b)
+        assert np.array_equal(x_level, oracle.substitute_lanes(rp, ci, dg, ofac, b, g)), g
+        for mode in (2, 3):
+            ilu.SetSweepMode(mode, mode, 512)
+            for _ in range(2):
+                assert np.array_equal(ilu.Substitute(b), x_level), (g, mode)
+    assert ilu.Schedule()["pack_l"]["max_width"] > 64
+
+
 @pytest.mark.parametrize("lof,closed_form", [(0, lambda s: 3 * s - 2), (1, lambda s: 6 * s - 5)])
 def test_schedule_levels_closed_form(oracle, lof, closed_form):
     """Levels of the L and U dependency graphs of a 3D 7-point stencil:
