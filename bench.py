@@ -55,12 +55,12 @@ def parse():
     p.add_argument("--forms", default="fp64,fp32,fp32-learned")
     p.add_argument("--cpu-steps", type=int, default=50,
                    help="Arnoldi steps in the bounded CPU baseline sample (0 = skip)")
-    p.add_argument("--sweep", default="3:8:3:8:4096",
+    p.add_argument("--sweep", default="",
                    help="sweep shape MODE_L:LANES_L:MODE_U:LANES_U:BLOCK, BLOCK = rows of contiguous blocks or a "
                         "tile TXxTYxTZ; 'auto' times every shape after the first factorization (untimed setup); "
-                        "'default' keeps the library's level-scheduled default.  The default is the shape the "
-                        "autotuner picks on the 128^3 workload (pipelined packed sweeps with LDS value-flags, "
-                        "8 lanes per row, 4096-row blocks)")
+                        "'default' keeps the library's level-scheduled default.  Without the option: the shape "
+                        "the autotuner picks on the 128^3 workload, pipelined packed sweeps with level barriers, "
+                        "8 lanes per row, tiles of min(side, 128) x 4 x 8 cells")
     p.add_argument("--traffic", default=str(ROOT / "profiles" / "sweep_traffic_convdiff128.txt"),
                    help="rocprofv3 summary holding the measured HBM bytes per apply")
     p.add_argument("--copy-calib", action="store_true",
@@ -96,6 +96,8 @@ def tri_bytes(n: int, nnz_f: int, value_bytes: int) -> int:
 
 def main():
     args = parse()
+    if not args.sweep:
+        args.sweep = f"2:8:2:8:{min(args.side, 128)}x4x8"
     if args.copy_calib:
         copy_calibration()
         return

@@ -964,9 +964,13 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     for (int blk : {4096, 8192, 16384})
         if (blk <= e->n) parts.push_back({blk, 0, 0, 0});
     if (e->grid[0] > 0) {
-        static const int kTiles[5][3] = {{32, 16, 16}, {24, 24, 24}, {16, 16, 32}, {16, 16, 16}, {32, 32, 16}};
+        // tiles spanning the whole x extent (up to 128) and several planes:
+        // few tiles on the long z and y dependency chains, ~4-8k rows each
+        const int tx = std::min(e->grid[0], 128);
+        const int kTiles[6][3] = {{tx, 4, 8}, {tx, 8, 6}, {tx, 6, 8}, {tx, 3, 10}, {tx, 8, 4}, {(tx * 3 + 3) / 4, 4, 8}};
         for (const auto& t : kTiles)
-            if (t[2] <= e->grid[2] && static_cast<long>(t[0]) * t[1] * t[2] < e->n)
+            if (t[1] <= e->grid[1] && t[2] <= e->grid[2] && static_cast<long>(t[0]) * t[1] * t[2] < e->n &&
+                static_cast<long>(t[0]) * t[1] * t[2] <= pilu::kMaxBlockRows)
                 parts.push_back({0, t[0], t[1], t[2]});
     }
     pilu::DevBuf<double> b(e->n), x(e->n);

@@ -191,6 +191,27 @@ def test_autotune_keeps_results(oracle):
     assert np.max(np.abs(x1 - x0)) <= 1e-12 * np.max(np.abs(x0))
 
 
+def test_autotune_with_grid_times_tiles(oracle):
+    """With a declared grid the autotuner also times tiles spanning the x
+    extent and several planes; whatever it keeps reproduces the sweep."""
+    s = 20
+    a = gen.convdiff3d(s)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
+    ilu = factored(o[0], o[1], o[3])
+    ilu.SetGridShape(s, s, s)
+    b = np.random.default_rng(11).uniform(-1, 1, a.n)
+    x0 = ilu.Substitute(b)
+    cfg, table = ilu.Autotune(reps=1)
+    tiles = table[table[:, 2] > 0]
+    assert {(s, 4, 8), (s, 8, 6), (15, 4, 8)} <= {tuple(int(v) for v in r[2:5]) for r in tiles}
+    assert np.all(table[:, 6:] > 0)
+    ofac, dg, _ = oracle.ilu_factorize(o[0], o[1], o[3])
+    x1 = ilu.Substitute(b)
+    assert np.max(np.abs(x1 - x0)) <= 1e-12 * np.max(np.abs(x0))
+    if cfg["lanes_l"] == cfg["lanes_u"]:
+        assert np.array_equal(x1, oracle.substitute_lanes(o[0], o[1], dg, ofac, b, cfg["lanes_l"]))
+
+
 @pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("blk", [64, 1000, 4096, 100000])
 @pytest.mark.parametrize("lanes", [1, 4, 8, 16])
