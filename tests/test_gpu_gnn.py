@@ -106,13 +106,15 @@ def test_learned_solve_fallback():
 @pytest.mark.parametrize("seed", [910_000, 910_001])
 def test_trained_network_beats_plain_ilu(seed):
     """Ablation with the shipped weights (trained on 64^3 GRF diffusion) on
-    unseen seeds of the same family at 40^3: the learned compensation needs
-    far fewer ILU(1)-GMRES(50) steps than plain ILU(1), without falling back,
-    and the FP64 true residual meets the tolerance.  A constant omega = 0.98
-    (the most frequent training label) is the non-learned reference point."""
+    unseen seeds of the same family and size: the learned compensation needs
+    far fewer ILU(1)-GMRES(50) steps than plain ILU(1) (measured on MI355X
+    for seeds 910000-910005: 0.39-0.47 of the plain count), without falling
+    back, and the FP64 true residual meets the tolerance.  A constant
+    omega = 0.98 (the most frequent training label) is the non-learned
+    reference point: the network must not do worse than it."""
     from par_ilu0_cpp_amd.learned import WEIGHTS
 
-    a = gen.grf_diffusion3d(40, sigma=1.5, corr_len=8.0, seed=seed)
+    a = gen.grf_diffusion3d(64, sigma=1.5, corr_len=8.0, seed=seed)
     b = gen.rhs_uniform(a.n, seed=seed)
     sys_ = DeviceSystem(a, b, lof=1, m=50)
     ok0, its0, res0 = sys_.solve_with_omega(None)
@@ -121,5 +123,5 @@ def test_trained_network_beats_plain_ilu(seed):
     ok, its, res, fb = sys_.solve_learned(net, budget=its0)
     assert ok0 and okc and ok and not fb
     assert res <= 1.01e-8 and res0 <= 1.01e-8
-    assert its <= 0.6 * its0, (its, its0)
+    assert its <= 0.55 * its0, (its, its0)
     assert its <= itsc + 2, (its, itsc)
