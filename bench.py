@@ -58,9 +58,10 @@ def parse():
     p.add_argument("--sweep", default="",
                    help="sweep shape MODE_L:LANES_L:MODE_U:LANES_U:BLOCK, BLOCK = rows of contiguous blocks or a "
                         "tile TXxTYxTZ; 'auto' times every shape after the first factorization (untimed setup); "
-                        "'default' keeps the library's level-scheduled default.  Without the option: the shape "
-                        "the autotuner picks on the 128^3 workload, pipelined packed sweeps with level barriers, "
-                        "8 lanes per row, tiles of min(side, 128) x 4 x 8 cells")
+                        "'default' keeps the library's level-scheduled default.  Without the option: the shapes "
+                        "the autotuner picks on the 128^3 workload, pipelined packed sweeps with 8 lanes per row on "
+                        "tiles of min(side, 128) x 4 x 8 cells, level barriers for FP64 factors (mode 2) and LDS "
+                        "value-flags for FP32 factors (mode 3)")
     p.add_argument("--traffic", default=str(ROOT / "profiles" / "sweep_traffic_convdiff128.txt"),
                    help="rocprofv3 summary holding the measured HBM bytes per apply")
     p.add_argument("--copy-calib", action="store_true",
@@ -96,8 +97,6 @@ def tri_bytes(n: int, nnz_f: int, value_bytes: int) -> int:
 
 def main():
     args = parse()
-    if not args.sweep:
-        args.sweep = f"2:8:2:8:{min(args.side, 128)}x4x8"
     if args.copy_calib:
         copy_calibration()
         return
@@ -157,8 +156,10 @@ def main():
         log(f"[bench] {form}: symbolic+analysis {t_setup*1e3:.1f} ms nnz(F)={nnz_f} levels L={nl} U={nu}")
         sweep_cfg = ilu.SweepConfig()
         ilu.SetGridShape(s, s, s)  # the workload's grid: lets the sweeps use 3D tiles
-        if args.sweep not in ("auto", "default"):
-            ml, ll, mu, lu, blk = args.sweep.split(":")
+        m = 2 if bits == 64 else 3
+        spec = args.sweep or f"{m}:8:{m}:8:{min(s, 128)}x4x8"
+        if spec not in ("auto", "default"):
+            ml, ll, mu, lu, blk = spec.split(":")
             ilu.SetSweepLanes(int(ll), int(lu))
             if "x" in blk:
                 ilu.SetSweepTiles(int(ml), int(mu), *(int(v) for v in blk.split("x")))
@@ -166,7 +167,7 @@ def main():
                 ilu.SetSweepMode(int(ml), int(mu), int(blk))
             sweep_cfg = ilu.SweepConfig()
             log(f"[bench] {form}: fixed sweep shape {sweep_cfg}")
-        elif args.sweep == "auto":
+        elif spec == "auto":
             # one factorization, then time every sweep shape (part of setup, untimed)
             if ilu.FactorizeDevice(d_va.data_ptr(), True) != 0:
                 raise SystemExit("factorization failed")

@@ -56,6 +56,7 @@ struct lljbash::IluSolver::Ext {
     int n = 0;
     long nnz = 0;
     int max_row = 0;
+    int max_lo = 0, max_up = 0;  // most strictly lower / upper entries in a row
     hipStream_t stream = nullptr;
 
     pilu::DevBuf<int> rp, ci, dg;

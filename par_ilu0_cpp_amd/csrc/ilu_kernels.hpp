@@ -692,6 +692,143 @@ __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict_
     retire_ticket(ctr);
 }
 
+// The same factorization for short rows: at most P pivots per row, at most G
+// strictly upper entries per pivot row (one per lane), rows of at most CAP
+// entries.  The arithmetic and its order are k_factor's (lane t owns entry
+// dk+1+t of every pivot row, pivots in ascending order, the same fma and
+// the same lane-strided sum of dropped updates), so the factors are
+// identical.  What changes is the waiting: the pivot rows' bounds, columns
+// and target slots come from the static pattern and are all fetched before
+// any wait, and the values of every pivot row are polled together, so a row
+// costs one round trip after its last pivot row is published instead of two
+// round trips per pivot.
+template <int G, int CAP, int P>
+__global__ __launch_bounds__(kBlock) void k_factor_short(int n, const int* __restrict__ perm,
+                                                         const int* __restrict__ rp,
+                                                         const int* __restrict__ ci,
+                                                         const int* __restrict__ dg,
+                                                         const double* __restrict__ vin, double* vout,
+                                                         float* f32, int* err, int* ctr, uint64_t wd_limit,
+                                                         const float* __restrict__ omega) {
+    constexpr int R = 64 / G;
+    constexpr int GROUPS = kBlock / G;
+    __shared__ int s_col[GROUPS][CAP];
+    __shared__ double s_val[GROUPS][CAP];
+    const int ticket = take_ticket(ctr);
+    const Watchdog wd(wd_limit);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int slot = lane / G;
+    const int grp = wave * R + slot;
+    const int t = lane & (G - 1);
+    const long pos = static_cast<long>(ticket) * GROUPS + grp;
+    const bool active = pos < n;
+    int* col = s_col[grp];
+    double* val = s_val[grp];
+    int row = 0, rb = 0, len = 0, ndiag = 0;
+    if (active) {
+        row = perm[pos];
+        rb = rp[row];
+        len = rp[row + 1] - rb;
+        ndiag = dg[row] - rb;
+        for (int q = t; q < len; q += G) {
+            col[q] = ci[rb + q];
+            val[q] = vin[rb + q];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // pattern of the pivot rows: diagonal position, this lane's entry and
+    // its slot in row i (-1: outside the pattern, -2: no entry)
+    int dk[P], qe[P], tgt[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        dk[p] = 0;
+        qe[p] = 0;
+        tgt[p] = -2;
+        if (active && p < ndiag) {
+            const int k = col[p];
+            dk[p] = dg[k];
+            const int ke = rp[k + 1];
+            const int q = dk[p] + 1 + t;
+            if (q < ke) {
+                qe[p] = q;
+                const int j = ci[q];
+                int lo = p + 1, hi = len;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (col[mid] < j) lo = mid + 1; else hi = mid;
+                }
+                tgt[p] = (lo < len && col[lo] == j) ? lo : -1;
+            }
+        }
+    }
+    uint64_t ub[P], uu[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) ub[p] = uu[p] = kPending;
+    bool done = !active;
+    while (true) {
+        if (!done) {
+            bool ok = true;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                if (p < ndiag) {
+                    if (ub[p] == kPending) ub[p] = peek_bits(vout + dk[p]);
+                    if (tgt[p] != -2 && uu[p] == kPending) uu[p] = peek_bits(vout + qe[p]);
+                    ok = ok && ub[p] != kPending && (tgt[p] == -2 || uu[p] != kPending);
+                }
+            }
+            bool ready = group_all<G>(ok);
+            if (!ready && group_all<G>(wd.expired())) {
+                if (t == 0) atomicOr(err, kErrTimeout);
+                for (int q = t; q < len; q += G) val[q] = __longlong_as_double(kQuietNan);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+                for (int p = 0; p < P; ++p) ub[p] = uu[p] = kQuietNan;
+                ndiag = 0;
+                ready = true;
+            }
+            if (ready) {
+                double dropped = 0.0;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    if (p < ndiag) {
+                        const double ukk = __longlong_as_double(ub[p]);
+                        const double lik = val[p] / ukk;
+                        if (ukk == 0.0 && t == 0) atomicOr(err, 1);
+                        const double u = __longlong_as_double(uu[p]);
+                        if (tgt[p] >= 0) val[tgt[p]] = fma(-u, lik, val[tgt[p]]);
+                        else if (tgt[p] == -1) dropped = fma(-u, lik, dropped);
+                        if (t == 0) val[p] = lik;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    }
+                }
+                if (omega) {
+                    const double dsum = group_sum<G>(dropped);
+                    if (t == 0) val[ndiag] = fma(static_cast<double>(omega[row]), dsum, val[ndiag]);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
+                for (int q = t; q < len; q += G) {
+                    const double v = val[q];
+                    if (f32) f32[rb + q] = static_cast<float>(v);
+                    publish(vout + rb + q, v);
+                }
+                done = true;
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        spin_pause();
+    }
+    retire_ticket(ctr);
+}
+
 // Zero the values to factor and arm the factor output with kPending.
 __global__ void k_prepare_factor(long nnz, double* __restrict__ vin, uint64_t* __restrict__ vout,
                                  bool zero_in) {

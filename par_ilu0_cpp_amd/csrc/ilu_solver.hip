@@ -389,15 +389,24 @@ void analyse(Ext* e, const int* host_rp) {
     if (flags[0]) throw std::runtime_error("rows must have strictly increasing, non-negative column indices");
     if (flags[1]) throw std::runtime_error("missing diag");
 
-    int max_row = 0;
-    if (host_rp) {
-        for (int i = 0; i < n; ++i) max_row = std::max(max_row, host_rp[i + 1] - host_rp[i]);
-    } else {
-        std::vector<int> h(static_cast<size_t>(n) + 1);
-        PILU_HIP(hipMemcpy(h.data(), e->rp.get(), h.size() * sizeof(int), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) max_row = std::max(max_row, h[i + 1] - h[i]);
+    int max_row = 0, max_lo = 0, max_up = 0;
+    {
+        std::vector<int> h, hd(n);
+        if (!host_rp) {
+            h.resize(static_cast<size_t>(n) + 1);
+            PILU_HIP(hipMemcpy(h.data(), e->rp.get(), h.size() * sizeof(int), hipMemcpyDeviceToHost));
+        }
+        const int* r = host_rp ? host_rp : h.data();
+        PILU_HIP(hipMemcpy(hd.data(), e->dg.get(), sizeof(int) * static_cast<size_t>(n), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            max_row = std::max(max_row, r[i + 1] - r[i]);
+            max_lo = std::max(max_lo, hd[i] - r[i]);
+            max_up = std::max(max_up, r[i + 1] - hd[i] - 1);
+        }
     }
     e->max_row = max_row;
+    e->max_lo = max_lo;
+    e->max_up = max_up;
 
     e->nlev_l = build_schedule(e, false, e->perm_l);
     e->nlev_u = build_schedule(e, true, e->perm_u);
@@ -714,6 +723,16 @@ int effective_lanes(const Ext* e, bool upper) {
     return lanes_for_avg(tri);
 }
 
+template <int G, int CAP, int P>
+void factor_short_one(Ext* e, float* f32) {
+    constexpr int groups = kBlock / G;
+    const int grid = grid_for(e->n, groups);
+    k_factor_short<G, CAP, P><<<grid, kBlock, 0, e->stream>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
+                                                             e->dg.get(), e->vin.get(), e->fac.get(), f32,
+                                                             e->err.get() + kErrFactor, e->ctr.get() + 2,
+                                                             watchdog_ticks(kWatchdogSeconds), e->omega_ptr);
+}
+
 template <int G, int CAP>
 void factor_one(Ext* e, float* f32) {
     constexpr int groups = kBlock / G;
@@ -734,7 +753,10 @@ bool run_factor(Ext* e) {
     }
     PILU_HIP(hipMemsetAsync(e->err.get() + kErrFactor, 0, sizeof(int), s));
     const int mr = e->max_row;
-    if (mr <= 32) factor_one<8, 32>(e, f32);
+    // short rows (stencils up to ILU(1) of a 3D 7-point operator): every
+    // pivot row fetched at once, one entry per lane
+    if (mr <= 32 && e->max_lo <= 8 && e->max_up <= 8) factor_short_one<8, 32, 8>(e, f32);
+    else if (mr <= 32) factor_one<8, 32>(e, f32);
     else if (mr <= 64) factor_one<16, 64>(e, f32);
     else if (mr <= 256) factor_one<32, 256>(e, f32);
     else if (mr <= 2560) factor_one<64, 2560>(e, f32);
