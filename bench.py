@@ -64,6 +64,10 @@ def parse():
                         "value-flags for FP32 factors (mode 3)")
     p.add_argument("--traffic", default=str(ROOT / "profiles" / "sweep_traffic_convdiff128.txt"),
                    help="rocprofv3 summary holding the measured HBM bytes per apply")
+    p.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"),
+                   help="c2: the benchmark line (128^3 convection-diffusion); c3: random n=1e6 ILU(2)-GMRES(40); "
+                        "c4: 512 batched 2D diffusion systems x 8 right-hand sides, ILU(0)-GMRES(30).  c3 and c4 "
+                        "print one JSON line of their own")
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a 1 GiB device copy three times (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
@@ -104,6 +108,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "c3":
+        workload_c3()
+        return
+    if args.workload == "c4":
+        workload_c4()
+        return
 
     import numpy as np
     import torch
@@ -282,6 +292,113 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def workload_c3(n=1_000_000, lof=2, m=40):
+    """Random n = 1e6 matrix (12 off-diagonals per row within +-5000,
+    diagonal 1.05 x the row's absolute sum), ILU(2), GMRES(40) to relative
+    residual 1e-8 from x0 = 0, FP64 and FP32 factors, default sweeps; one
+    solve each, with the FP32 solution's relative distance to the FP64 one."""
+    import numpy as np
+    import torch
+
+    import par_ilu0_cpp_amd as P
+    from par_ilu0_cpp_amd import generators as gen
+
+    t0 = time.time()
+    a = gen.random_banded(n, 12, 5000, seed=0)
+    b = gen.rhs_uniform(n, seed=0)
+    log(f"[c3] generated n={n} nnz={a.nnz} in {time.time() - t0:.1f}s")
+    d = {k: torch.from_numpy(v).cuda() for k, v in (("rp", a.row_ptr), ("ci", a.col_idx), ("va", a.value), ("b", b))}
+    out = {"workload": "random banded n=1e6, 12 off-diagonals within +-5000", "n": n, "nnz_a": a.nnz, "lof": lof,
+           "m": m, "data": "synthetic (seed 0)"}
+    sols = {}
+    for bits in (64, 32):
+        ilu = P.IluSolver(0)
+        ilu.SetFactorPrecision(bits)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        nnz_f = ilu.SymbolicDevice(n, d["rp"].data_ptr(), d["ci"].data_ptr(), lof)
+        torch.cuda.synchronize()
+        t_sym = time.perf_counter() - ts
+        if nnz_f <= 0:
+            raise SystemExit(f"symbolic failed ({nnz_f})")
+        ts = time.perf_counter()
+        if ilu.FactorizeDevice(d["va"].data_ptr(), True) != 0:
+            raise SystemExit("factorization failed")
+        torch.cuda.synchronize()
+        t_fac = time.perf_counter() - ts
+        g = P.PreconditionedGmres()
+        g.SetIluPreconditioner(ilu)
+        prm = g.param()
+        prm.tolerance = 1e-8 * float(np.linalg.norm(b))
+        prm.krylov_subspace_dimension = m
+        prm.max_iterations = 2000
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        ok, its = g.SolveDevice(n, d["rp"].data_ptr(), d["ci"].data_ptr(), d["va"].data_ptr(), d["b"].data_ptr(),
+                                x.data_ptr())
+        torch.cuda.synchronize()
+        t_sol = time.perf_counter() - ts
+        sols[bits] = x.cpu().numpy()
+        out[f"fp{bits}"] = {"nnz_f": nnz_f, "levels": list(ilu.Levels()), "symbolic_s": t_sym, "factor_s": t_fac,
+                            "solve_s": t_sol, "its": its, "ok": bool(ok),
+                            "true_rel_res": g.LastTrueResidual() / float(np.linalg.norm(b)),
+                            "sweep": ilu.SweepConfig()}
+        log(f"[c3] fp{bits}: {out[f'fp{bits}']}")
+        del g, ilu
+    out["fp32_vs_fp64_rel_err"] = float(np.linalg.norm(sols[32] - sols[64]) / np.linalg.norm(sols[64]))
+    print(json.dumps(out), flush=True)
+
+
+def workload_c4(count=512, nrhs=8, m=30):
+    """512 uneven 2D diffusion systems (sides 45..256, lognormal
+    coefficients) x 8 uniform right-hand sides, block-diagonal ILU(0),
+    GMRES(30) to relative residual 1e-8 per pair, FP64 and FP32 factors;
+    the second of two solves is timed."""
+    import numpy as np
+    import torch
+
+    import par_ilu0_cpp_amd as P
+    from par_ilu0_cpp_amd import generators as gen
+
+    t0 = time.time()
+    mats = gen.batched_diffusion2d(count, seed=0)
+    rp, ci, va, sp = P.block_diagonal(mats)
+    n = rp.size - 1
+    rhs = np.random.default_rng(0).uniform(-1, 1, size=(n, nrhs))
+    log(f"[c4] generated {count} systems, sum n={n}, nnz={ci.size} in {time.time() - t0:.1f}s")
+    d = [torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (rp, ci, va, rhs)]
+    x = torch.zeros((n, nrhs), dtype=torch.float64, device="cuda")
+    out = {"workload": "batched 2D diffusion", "systems": count, "nrhs": nrhs, "sum_n": n, "nnz": int(ci.size),
+           "m": m, "data": "synthetic (seed 0)"}
+    for bits in (64, 32):
+        bg = P.BatchGmres()
+        bg.SetFactorPrecision(bits)
+        ts = time.perf_counter()
+        if bg.Setup(n, d[0].data_ptr(), d[1].data_ptr(), 0, sp, nrhs) <= 0:
+            raise SystemExit("batched setup failed")
+        t_set = time.perf_counter() - ts
+        ts = time.perf_counter()
+        if bg.Factorize(d[2].data_ptr()) != 0:
+            raise SystemExit("batched factorization failed")
+        t_fac = time.perf_counter() - ts
+        for _ in range(2):
+            x.zero_()
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            cyc, its, conv, rel = bg.Solve(d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr(),
+                                           x.data_ptr(), 1e-8, m, 2000)
+            torch.cuda.synchronize()
+            t_sol = time.perf_counter() - ts
+        out[f"fp{bits}"] = {"setup_s": t_set, "factor_s": t_fac, "solve_s": t_sol, "cycles": cyc,
+                            "its_mean": float(its.mean()), "its_max": int(its.max()),
+                            "converged": int(conv.sum()), "pairs": int(conv.size),
+                            "max_rel_res": float(rel.max()), "pair_solves_per_s": conv.size / t_sol}
+        log(f"[c4] fp{bits}: {out[f'fp{bits}']}")
+        del bg
+    print(json.dumps(out), flush=True)
 
 
 def traffic_bytes(path, form):

@@ -68,8 +68,9 @@ public:
     // Times the sweeps in every launch shape -- level-scheduled rows with
     // 1..64 lanes per row, and block-local sweeps over contiguous blocks of
     // 4096, 8192 or 16384 rows (CSR or packed factors) and, once the grid
-    // shape is declared, over 3D tiles (packed factors) -- over `reps`
-    // applies each, and keeps the fastest shape for each direction.
+    // shape is declared, over tiles spanning the x extent (up to 128 cells)
+    // and several planes (packed factors) -- over `reps` applies each, and
+    // keeps the fastest shape for each direction.
     // Results depend on the lanes per row only through the rounding of row
     // sums.  table (optional, cap rows of 8 doubles): mode, block rows, tile
     // x, y, z, lanes, forward ms, backward ms.  Returns the shapes timed.
@@ -77,12 +78,14 @@ public:
     // Sweep algorithm per direction: 0 level-scheduled rows, 1 block-local
     // over contiguous blocks of blk_rows rows (64..16384), 2 block-local over
     // a packed, schedule-ordered copy of the factors (refreshed at every
-    // Factorize) with its loads issued several passes ahead.
+    // Factorize) with its loads issued several passes ahead and a workgroup
+    // barrier per local level, 3 the same with the block's x held in LDS as
+    // value-flags instead of barriers.
     void SetSweepMode(int mode_l, int mode_u, int blk_rows);
     // Declares that row x + nx (y + ny z) is grid cell (x, y, z); enables
     // tiled partitions of the block-local sweeps.
     void SetGridShape(int nx, int ny, int nz);
-    // Block-local sweeps (mode 0 or 2 per direction) over tx x ty x tz
+    // Block-local sweeps (mode 0, 2 or 3 per direction) over tx x ty x tz
     // tiles of the declared grid (at most 16384 rows per tile).  Blocks run
     // in a topological order of their dependency graph; the results are
     // those of the same lanes per row with any other partition.

@@ -35,9 +35,10 @@ int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
 
 /* Autotune the sweeps after Factorize: level-scheduled rows with 1..64
  * lanes per row and the block-local sweeps (contiguous blocks of
- * 4096/8192/16384 rows, and 3D tiles once the grid shape is declared; CSR or
- * packed factors) with 2..32 lanes are each timed over `reps` applies; the
- * fastest shape per direction is kept.  table (optional, cap rows x 8
+ * 4096/8192/16384 rows, and, once the grid shape is declared, tiles spanning
+ * the x extent (up to 128 cells) and several planes; CSR factors, or packed
+ * factors with level barriers or LDS value-flags) with 2..32 lanes are each
+ * timed over `reps` applies; the fastest shape per direction is kept.  table (optional, cap rows x 8
  * doubles): mode, block rows, tile x, tile y, tile z, lanes, forward ms,
  * backward ms.  Returns the rows timed. */
 #define IluSolverAutotuneEx LLJBASH_DECL(IluSolverAutotuneEx)
@@ -45,7 +46,9 @@ int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap);
 
 /* Explicit sweep algorithm per direction (0 level rows, 1 block-local over
  * contiguous blocks of blk_rows rows, 2 block-local over packed factors,
- * software-pipelined) and the current configuration (cfg[8]: mode_l,
+ * software-pipelined, with a workgroup barrier per local level, 3 the same
+ * with the block's x as LDS value-flags and no barriers) and the current
+ * configuration (cfg[8]: mode_l,
  * lanes_l, mode_u, lanes_u, blk_rows, tile x, tile y, tile z). */
 #define IluSolverSetSweepModeEx LLJBASH_DECL(IluSolverSetSweepModeEx)
 int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_rows);
@@ -53,8 +56,8 @@ int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_ro
  * nx * ny * nz must equal n.  Enables tiled block-local sweeps. */
 #define IluSolverSetGridShapeEx LLJBASH_DECL(IluSolverSetGridShapeEx)
 int IluSolverSetGridShapeEx(IluSolverHdl hdl, int nx, int ny, int nz);
-/* Block-local sweeps (mode 0 or 2 per direction) over tx x ty x tz tiles of
- * the declared grid (at most 16384 rows per tile). */
+/* Block-local sweeps (mode 0, 2 or 3 per direction) over tx x ty x tz tiles
+ * of the declared grid (at most 16384 rows per tile). */
 #define IluSolverSetSweepTilesEx LLJBASH_DECL(IluSolverSetSweepTilesEx)
 int IluSolverSetSweepTilesEx(IluSolverHdl hdl, int mode_l, int mode_u, int tx, int ty, int tz);
 #define IluSolverGetSweepConfigEx LLJBASH_DECL(IluSolverGetSweepConfigEx)
