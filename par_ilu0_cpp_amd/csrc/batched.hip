@@ -39,11 +39,22 @@ struct BCtl {
 };
 
 // ---- multi-RHS sweep: lane r of a K-lane group carries RHS r of one row ----
+// Rows in level order of the block-diagonal factor, value-flags as in the
+// single-system sweep.  Rows of a system with no instance running the
+// current Arnoldi step (row_sys/ctl) are skipped: dependencies never cross
+// systems, and the liveness is the same for the L and U launches of one
+// apply, so the skipped rows' buffers stay armed for the next live apply.
+// The first kMsE entries of a row are fetched at once and their x polled
+// together; longer rows continue one entry at a time.  The sum runs in entry
+// order from b either way.
+constexpr int kMsE = 4;
+
 template <int K, typename T, bool UPPER>
 __global__ __launch_bounds__(kBThreads) void k_msweep(int n, const int* __restrict__ perm, const int* __restrict__ rp,
                                                       const int* __restrict__ ci, const int* __restrict__ dg,
                                                       const T* __restrict__ val, const double* b, double* out,
-                                                      double* reset, int* ctr, int* err, uint64_t wd_limit) {
+                                                      double* reset, int* ctr, int* err, uint64_t wd_limit,
+                                                      const int* __restrict__ row_sys, const BCtl* ctl) {
     constexpr int R = 64 / K;
     const int ticket = take_ticket(ctr);
     const Watchdog wd(wd_limit);
@@ -51,33 +62,75 @@ __global__ __launch_bounds__(kBThreads) void k_msweep(int n, const int* __restri
     const int wave = threadIdx.x >> 6;
     const int r = lane % K;
     const long pos = (static_cast<long>(ticket) * (kBThreads / 64) + wave) * R + lane / K;
-    const bool active = pos < n;
+    bool active = pos < n;
     int row = 0, e = 0, end = 0;
-    double acc = 0.0, dval = 1.0;
     if (active) {
         row = perm[pos];
         const int d = dg[row];
         e = UPPER ? d + 1 : rp[row];
         end = UPPER ? rp[row + 1] : d;
-        if (UPPER) dval = static_cast<double>(val[d]);
+    }
+    if (ctl) {
+        const bool mine = active && ctl[static_cast<long>(row_sys[row]) * K + r].step_active != 0;
+        const uint64_t m = __ballot(mine);
+        active = active && ((m >> ((lane / K) * K)) & ((K == 64 ? 0ull : (1ull << K)) - 1ull)) != 0;
+    }
+    double acc = 0.0, dval = 1.0;
+    int c[kMsE];
+    T v[kMsE];
+    uint64_t xb[kMsE];
+#pragma unroll
+    for (int q = 0; q < kMsE; ++q) {
+        c[q] = 0;
+        v[q] = T(0);
+        xb[q] = kPending;
+    }
+    if (active) {
+        if (UPPER) dval = static_cast<double>(val[dg[row]]);
+#pragma unroll
+        for (int q = 0; q < kMsE; ++q)
+            if (e + q < end) {
+                c[q] = ci[e + q];
+                v[q] = val[e + q];
+            }
         acc = b[static_cast<long>(row) * K + r];
         if (reset) reset[static_cast<long>(row) * K + r] = __longlong_as_double(kPending);
     }
+    bool head = true;  // the first kMsE entries are still to be summed
     bool done = !active;
     while (true) {
         if (!done) {
-            while (e < end) {
-                const uint64_t bits = peek_bits(out + static_cast<long>(ci[e]) * K + r);
-                if (bits == kPending) break;
-                acc = fma(-__longlong_as_double(bits), static_cast<double>(val[e]), acc);
-                ++e;
+            if (head) {
+                bool ok = true;
+#pragma unroll
+                for (int q = 0; q < kMsE; ++q)
+                    if (e + q < end) {
+                        if (xb[q] == kPending) xb[q] = peek_bits(out + static_cast<long>(c[q]) * K + r);
+                        ok = ok && xb[q] != kPending;
+                    }
+                if (ok) {
+#pragma unroll
+                    for (int q = 0; q < kMsE; ++q)
+                        if (e + q < end) acc = fma(-__longlong_as_double(xb[q]), static_cast<double>(v[q]), acc);
+                    e = e + kMsE < end ? e + kMsE : end;
+                    head = false;
+                }
             }
-            if (e < end && wd.expired()) {
+            if (!head) {
+                while (e < end) {
+                    const uint64_t bits = peek_bits(out + static_cast<long>(ci[e]) * K + r);
+                    if (bits == kPending) break;
+                    acc = fma(-__longlong_as_double(bits), static_cast<double>(val[e]), acc);
+                    ++e;
+                }
+            }
+            if ((head || e < end) && wd.expired()) {
                 atomicOr(err, kErrTimeout);
                 acc = __longlong_as_double(kQuietNan);
                 e = end;
+                head = false;
             }
-            if (e >= end) {
+            if (!head && e >= end) {
                 publish(out + static_cast<long>(row) * K + r, UPPER ? acc / dval : acc);
                 done = true;
             }
@@ -415,6 +468,11 @@ public:
         PILU_HIP(hipMemcpy(chunk_beg_.get(), beg.data(), beg.size() * sizeof(int), hipMemcpyHostToDevice));
         PILU_HIP(hipMemcpy(chunk_sys_.get(), sys.data(), sys.size() * sizeof(int), hipMemcpyHostToDevice));
         PILU_HIP(hipMemcpy(sys_chunk_.get(), sc.data(), sc.size() * sizeof(int), hipMemcpyHostToDevice));
+        // system of every row (the sweeps skip rows of systems with no live instance)
+        std::vector<int> rs(static_cast<size_t>(n_));
+        for (int q = 0; q < nsys; ++q) std::fill(rs.begin() + sys_ptr[q], rs.begin() + sys_ptr[q + 1], q);
+        row_sys_.resize(rs.size());
+        PILU_HIP(hipMemcpy(row_sys_.get(), rs.data(), rs.size() * sizeof(int), hipMemcpyHostToDevice));
         // pattern: symbolic ILU(lof) of the block-diagonal matrix
         const int nnz_f = ilu_.SymbolicDevice(n_, d_a->row_ptr, d_a->col_idx, lof);
         if (nnz_f < 0) return nnz_f;
@@ -544,10 +602,11 @@ private:
         const uint64_t wd = watchdog_ticks(10.0);
         k_msweep<K, T, false><<<grid, kBThreads, 0, stream_>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
                                                                e->dg.get(), val, b, y_.get(), x, e->ctr.get() + 8,
-                                                               e->err.get() + 1, wd);
+                                                               e->err.get() + 1, wd, row_sys_.get(), ctl_.get());
         k_msweep<K, T, true><<<grid, kBThreads, 0, stream_>>>(e->n, e->perm_u.get(), e->rp.get(), e->ci.get(),
                                                               e->dg.get(), val, y_.get(), x, y_.get(),
-                                                              e->ctr.get() + 10, e->err.get() + 1, wd);
+                                                              e->ctr.get() + 10, e->err.get() + 1, wd, row_sys_.get(),
+                                                              ctl_.get());
     }
 
     template <int K>
@@ -640,7 +699,7 @@ private:
                *grp_ = nullptr, *gci_ = nullptr;
     int K_ = 8, nsys_ = 0, n_ = 0, nchunks_ = 0, m_ = -1, maxits_ = -1, cycles_ = 0;
     bool ready_ = false;
-    DevBuf<int> chunk_beg_, chunk_sys_, sys_chunk_, flags_, int_out_;
+    DevBuf<int> chunk_beg_, chunk_sys_, sys_chunk_, row_sys_, flags_, int_out_;
     DevBuf<double> v_, z_, hh_, part_, y_, res_, dbl_out_;
     DevBuf<BCtl> ctl_;
 };
