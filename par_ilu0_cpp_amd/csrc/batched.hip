@@ -38,107 +38,44 @@ struct BCtl {
     double tol, beta, inv, pad;
 };
 
-// ---- multi-RHS sweep: lane r of a K-lane group carries RHS r of one row ----
-// Rows in level order of the block-diagonal factor, value-flags as in the
-// single-system sweep.  Rows of a system with no instance running the
-// current Arnoldi step (row_sys/ctl) are skipped: dependencies never cross
-// systems, and the liveness is the same for the L and U launches of one
-// apply, so the skipped rows' buffers stay armed for the next live apply.
-// The first kMsE entries of a row are fetched at once and their x polled
-// together; longer rows continue one entry at a time.  The sum runs in entry
-// order from b either way.
-constexpr int kMsE = 4;
+// ---- multi-RHS sweep: one workgroup per system ----
+// Workgroup s walks the (level, row) groups of system s in order (perm,
+// sys_grp, grp_off from block_level_schedule with the systems as blocks);
+// the K right-hand sides of a row are K adjacent lanes.  Rows of one level
+// are independent and the levels are separated by a workgroup barrier, so
+// no cross-workgroup waits are needed.  Each lane sums its row in entry
+// order from b, one fma per entry, as the single-system sweep with one lane
+// per row does.  A system with no instance running the current Arnoldi step
+// returns at once.
+constexpr int kSThreads = 1024;
 
 template <int K, typename T, bool UPPER>
-__global__ __launch_bounds__(kBThreads) void k_msweep(int n, const int* __restrict__ perm, const int* __restrict__ rp,
+__global__ __launch_bounds__(kSThreads) void k_ssweep(const int* __restrict__ sys_grp, const int* __restrict__ grp_off,
+                                                      const int* __restrict__ perm, const int* __restrict__ rp,
                                                       const int* __restrict__ ci, const int* __restrict__ dg,
                                                       const T* __restrict__ val, const double* b, double* out,
-                                                      double* reset, int* ctr, int* err, uint64_t wd_limit,
-                                                      const int* __restrict__ row_sys, const BCtl* ctl) {
-    constexpr int R = 64 / K;
-    const int ticket = take_ticket(ctr);
-    const Watchdog wd(wd_limit);
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int r = lane % K;
-    const long pos = (static_cast<long>(ticket) * (kBThreads / 64) + wave) * R + lane / K;
-    bool active = pos < n;
-    int row = 0, e = 0, end = 0;
-    if (active) {
-        row = perm[pos];
-        const int d = dg[row];
-        e = UPPER ? d + 1 : rp[row];
-        end = UPPER ? rp[row + 1] : d;
-    }
+                                                      const BCtl* ctl) {
+    const int s = blockIdx.x;
     if (ctl) {
-        const bool mine = active && ctl[static_cast<long>(row_sys[row]) * K + r].step_active != 0;
-        const uint64_t m = __ballot(mine);
-        active = active && ((m >> ((lane / K) * K)) & ((K == 64 ? 0ull : (1ull << K)) - 1ull)) != 0;
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < K; ++r) any = any || ctl[s * K + r].step_active != 0;
+        if (!any) return;
     }
-    double acc = 0.0, dval = 1.0;
-    int c[kMsE];
-    T v[kMsE];
-    uint64_t xb[kMsE];
-#pragma unroll
-    for (int q = 0; q < kMsE; ++q) {
-        c[q] = 0;
-        v[q] = T(0);
-        xb[q] = kPending;
-    }
-    if (active) {
-        if (UPPER) dval = static_cast<double>(val[dg[row]]);
-#pragma unroll
-        for (int q = 0; q < kMsE; ++q)
-            if (e + q < end) {
-                c[q] = ci[e + q];
-                v[q] = val[e + q];
-            }
-        acc = b[static_cast<long>(row) * K + r];
-        if (reset) reset[static_cast<long>(row) * K + r] = __longlong_as_double(kPending);
-    }
-    bool head = true;  // the first kMsE entries are still to be summed
-    bool done = !active;
-    while (true) {
-        if (!done) {
-            if (head) {
-                bool ok = true;
-#pragma unroll
-                for (int q = 0; q < kMsE; ++q)
-                    if (e + q < end) {
-                        if (xb[q] == kPending) xb[q] = peek_bits(out + static_cast<long>(c[q]) * K + r);
-                        ok = ok && xb[q] != kPending;
-                    }
-                if (ok) {
-#pragma unroll
-                    for (int q = 0; q < kMsE; ++q)
-                        if (e + q < end) acc = fma(-__longlong_as_double(xb[q]), static_cast<double>(v[q]), acc);
-                    e = e + kMsE < end ? e + kMsE : end;
-                    head = false;
-                }
-            }
-            if (!head) {
-                while (e < end) {
-                    const uint64_t bits = peek_bits(out + static_cast<long>(ci[e]) * K + r);
-                    if (bits == kPending) break;
-                    acc = fma(-__longlong_as_double(bits), static_cast<double>(val[e]), acc);
-                    ++e;
-                }
-            }
-            if ((head || e < end) && wd.expired()) {
-                atomicOr(err, kErrTimeout);
-                acc = __longlong_as_double(kQuietNan);
-                e = end;
-                head = false;
-            }
-            if (!head && e >= end) {
-                publish(out + static_cast<long>(row) * K + r, UPPER ? acc / dval : acc);
-                done = true;
-            }
+    const int r = threadIdx.x % K;
+    for (int g = sys_grp[s]; g < sys_grp[s + 1]; ++g) {
+        const int p1 = grp_off[g + 1];
+        for (int p = grp_off[g] + static_cast<int>(threadIdx.x) / K; p < p1; p += kSThreads / K) {
+            const int row = perm[p];
+            const int d = dg[row];
+            const int end = UPPER ? rp[row + 1] : d;
+            double acc = b[static_cast<long>(row) * K + r];
+            for (int e = UPPER ? d + 1 : rp[row]; e < end; ++e)
+                acc = fma(-out[static_cast<long>(ci[e]) * K + r], static_cast<double>(val[e]), acc);
+            out[static_cast<long>(row) * K + r] = UPPER ? acc / static_cast<double>(val[d]) : acc;
         }
-        if (__ballot(!done) == 0) break;
-        spin_pause();
+        __syncthreads();
     }
-    retire_ticket(ctr);
 }
 
 // Sum over the threads of a block that share t % K; result valid in
@@ -428,10 +365,6 @@ __global__ void k_bresult(int ni, int K, const BCtl* ctl, const double* part, co
     rel[inst] = rb > 0 ? rr / rb : rr;
 }
 
-__global__ void k_fill_pending(long n, uint64_t* p) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = kPending;
-}
-
 }  // namespace
 
 class BatchedGmres {
@@ -468,7 +401,7 @@ public:
         PILU_HIP(hipMemcpy(chunk_beg_.get(), beg.data(), beg.size() * sizeof(int), hipMemcpyHostToDevice));
         PILU_HIP(hipMemcpy(chunk_sys_.get(), sys.data(), sys.size() * sizeof(int), hipMemcpyHostToDevice));
         PILU_HIP(hipMemcpy(sys_chunk_.get(), sc.data(), sc.size() * sizeof(int), hipMemcpyHostToDevice));
-        // system of every row (the sweeps skip rows of systems with no live instance)
+        // system of every row (the sweeps' schedule is per system)
         std::vector<int> rs(static_cast<size_t>(n_));
         for (int q = 0; q < nsys; ++q) std::fill(rs.begin() + sys_ptr[q], rs.begin() + sys_ptr[q + 1], q);
         row_sys_.resize(rs.size());
@@ -476,6 +409,10 @@ public:
         // pattern: symbolic ILU(lof) of the block-diagonal matrix
         const int nnz_f = ilu_.SymbolicDevice(n_, d_a->row_ptr, d_a->col_idx, lof);
         if (nnz_f < 0) return nnz_f;
+        // per-system level schedules of the sweeps
+        Ext* ie = ilu_.DeviceState();
+        block_level_schedule(ie, false, row_sys_.get(), nsys, sperm_l_, sgrp_l_, soff_l_);
+        block_level_schedule(ie, true, row_sys_.get(), nsys, sperm_u_, sgrp_u_, soff_u_);
         PILU_HIP(hipDeviceSynchronize());
         ready_ = false;
         return nnz_f;
@@ -520,7 +457,6 @@ public:
         double* pb = part_.get() + 2L * nchunks_ * K_;
         norm_launch(d_rhs, pb);
         k_binit<<<ceil_div(ni, 256), 256, 0, stream_>>>(ni, ctl_.get(), pb, sys_chunk_.get(), K_, rel_tol, maxits);
-        k_fill_pending<<<1024, 256, 0, stream_>>>(nk, reinterpret_cast<uint64_t*>(y_.get()));
         PILU_HIP(hipGetLastError());
         if (!graph_exec_) {
             PILU_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
@@ -597,16 +533,12 @@ private:
 
     template <int K, typename T>
     void sweeps_t(Ext* e, const T* val, const double* b, double* x) {
-        const int rows_per_block = (64 / K) * (kBThreads / 64);
-        const int grid = ceil_div(e->n, rows_per_block);
-        const uint64_t wd = watchdog_ticks(10.0);
-        k_msweep<K, T, false><<<grid, kBThreads, 0, stream_>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
-                                                               e->dg.get(), val, b, y_.get(), x, e->ctr.get() + 8,
-                                                               e->err.get() + 1, wd, row_sys_.get(), ctl_.get());
-        k_msweep<K, T, true><<<grid, kBThreads, 0, stream_>>>(e->n, e->perm_u.get(), e->rp.get(), e->ci.get(),
-                                                              e->dg.get(), val, y_.get(), x, y_.get(),
-                                                              e->ctr.get() + 10, e->err.get() + 1, wd, row_sys_.get(),
-                                                              ctl_.get());
+        k_ssweep<K, T, false><<<nsys_, kSThreads, 0, stream_>>>(sgrp_l_.get(), soff_l_.get(), sperm_l_.get(),
+                                                                e->rp.get(), e->ci.get(), e->dg.get(), val, b,
+                                                                y_.get(), ctl_.get());
+        k_ssweep<K, T, true><<<nsys_, kSThreads, 0, stream_>>>(sgrp_u_.get(), soff_u_.get(), sperm_u_.get(),
+                                                               e->rp.get(), e->ci.get(), e->dg.get(), val, y_.get(),
+                                                               x, ctl_.get());
     }
 
     template <int K>
@@ -700,6 +632,7 @@ private:
     int K_ = 8, nsys_ = 0, n_ = 0, nchunks_ = 0, m_ = -1, maxits_ = -1, cycles_ = 0;
     bool ready_ = false;
     DevBuf<int> chunk_beg_, chunk_sys_, sys_chunk_, row_sys_, flags_, int_out_;
+    DevBuf<int> sperm_l_, sgrp_l_, soff_l_, sperm_u_, sgrp_u_, soff_u_;  // per-system level schedules
     DevBuf<double> v_, z_, hh_, part_, y_, res_, dbl_out_;
     DevBuf<BCtl> ctl_;
 };

@@ -27,6 +27,14 @@ void check_sweep_errors(lljbash::IluSolver::Ext* ext);
 // heuristic; the autotuner may override it).
 int lanes_for_avg(double avg_len);
 
+// Level schedule of independent diagonal blocks (block_of[row] in
+// [0, nblk)): perm = rows sorted by (block, level inside the block, row),
+// blev[b] = first (block, level) group of block b (nblk + 1 entries),
+// loff[g] = first position of group g.  Levels of the L (upper = false) or
+// U dependency graph.
+void block_level_schedule(lljbash::IluSolver::Ext* ext, bool upper, const int* block_of, int nblk,
+                          DevBuf<int>& perm, DevBuf<int>& blev, DevBuf<int>& loff);
+
 }  // namespace pilu
 
 namespace pilu {

@@ -134,18 +134,18 @@ __global__ void k_block_keys(int n, const int* __restrict__ block_of, const int*
 // Schedule of the block-local sweeps in one direction: levels restricted to
 // each block's own rows, rows sorted by (processing position of their
 // block, local level, row), and the group offsets of every (block, level).
-// pos[b]: processing position of block b.
-void build_block_schedule(Ext* e, bool upper, const std::vector<int>& pos, DevBuf<int>& perm, DevBuf<int>& blev,
-                          DevBuf<int>& loff) {
+// block_of[row]: block of every row (nblk blocks); pos[b]: processing
+// position of block b.
+void build_block_schedule(Ext* e, bool upper, const int* block_of, int nblk, const std::vector<int>& pos,
+                          DevBuf<int>& perm, DevBuf<int>& blev, DevBuf<int>& loff) {
     const int n = e->n;
-    const int nblk = e->nblk;
     hipStream_t s = e->stream;
     int* level = e->level.get();
     int* maxlev = e->err.get() + kErrMaxLev;
     k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
     PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
     const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
-    const int* bo = e->block_of.get();
+    const int* bo = block_of;
     if (upper) launch_levels<true>(e, bo, level, maxlev, wd);
     else launch_levels<false>(e, bo, level, maxlev, wd);
     PILU_HIP(hipGetLastError());
@@ -284,8 +284,8 @@ void install_partition(Ext* e, int nblk, bool contiguous) {
         pos_l = block_order(e, rp, ci, dg, false);
         pos_u = block_order(e, rp, ci, dg, true);
     }
-    build_block_schedule(e, false, pos_l, e->bperm_l, e->blev_l, e->loff_l);
-    build_block_schedule(e, true, pos_u, e->bperm_u, e->blev_u, e->loff_u);
+    build_block_schedule(e, false, e->block_of.get(), e->nblk, pos_l, e->bperm_l, e->blev_l, e->loff_l);
+    build_block_schedule(e, true, e->block_of.get(), e->nblk, pos_u, e->bperm_u, e->blev_u, e->loff_u);
     ++e->part_version;  // packs are rebuilt on first use
 }
 
@@ -804,6 +804,17 @@ void ilu_apply(Ext* e, const double* d_b, double* d_x, const int* gate) {
         sweep_dispatch<double, true>(gu, e, e->fac.get(), e->y.get(), d_x, e->y.get(), gate, ctr + 2);
     }
     PILU_HIP(hipGetLastError());
+}
+
+// Schedule of independent diagonal blocks given by block_of (the systems of
+// a block-diagonal matrix): rows sorted by (block, level within the block,
+// row), blev[b] the first (block, level) group of block b, loff[g] the first
+// position of group g.
+void block_level_schedule(Ext* e, bool upper, const int* block_of, int nblk, DevBuf<int>& perm, DevBuf<int>& blev,
+                          DevBuf<int>& loff) {
+    std::vector<int> pos(static_cast<size_t>(nblk));
+    for (int b = 0; b < nblk; ++b) pos[b] = b;
+    build_block_schedule(e, upper, block_of, nblk, pos, perm, blev, loff);
 }
 
 }  // namespace pilu
