@@ -47,7 +47,60 @@ struct BCtl {
 // order from b, one fma per entry, as the single-system sweep with one lane
 // per row does.  A system with no instance running the current Arnoldi step
 // returns at once.
+//
+// Software pipeline: everything a row needs except the x of its
+// dependencies (its position, bounds, first kSsE columns and values, b and
+// the diagonal) comes from the pattern, the factors or the input vector, so
+// it is requested one level ahead for the first kSsSlots row slots of every
+// thread; a level then waits only for its x reads, which hit the lines this
+// workgroup wrote before the barrier.  Further slots and entries are read
+// directly.
 constexpr int kSThreads = 1024;
+constexpr int kSsE = 4;
+constexpr int kSsSlots = 2;
+
+template <typename T>
+struct SsRow {
+    int row, e, end;
+    int c[kSsE];
+    T v[kSsE];
+    T dv;
+    double bv;
+};
+
+template <int K, typename T, bool UPPER>
+__device__ __forceinline__ void ss_fetch(int p, int p1, int r, const int* __restrict__ perm, const int* __restrict__ rp,
+                                         const int* __restrict__ ci, const int* __restrict__ dg,
+                                         const T* __restrict__ val, const double* b, SsRow<T>& t) {
+    t.row = -1;
+    t.e = t.end = 0;
+    if (p >= p1) return;
+    t.row = perm[p];
+    const int d = dg[t.row];
+    t.e = UPPER ? d + 1 : rp[t.row];
+    t.end = UPPER ? rp[t.row + 1] : d;
+#pragma unroll
+    for (int q = 0; q < kSsE; ++q)
+        if (t.e + q < t.end) {
+            t.c[q] = ci[t.e + q];
+            t.v[q] = val[t.e + q];
+        }
+    if (UPPER) t.dv = val[d];
+    t.bv = b[static_cast<long>(t.row) * K + r];
+}
+
+template <int K, typename T, bool UPPER>
+__device__ __forceinline__ void ss_solve(const SsRow<T>& t, int r, const int* __restrict__ ci,
+                                         const T* __restrict__ val, double* out) {
+    if (t.row < 0) return;
+    double acc = t.bv;
+#pragma unroll
+    for (int q = 0; q < kSsE; ++q)
+        if (t.e + q < t.end) acc = fma(-out[static_cast<long>(t.c[q]) * K + r], static_cast<double>(t.v[q]), acc);
+    for (int e = t.e + kSsE; e < t.end; ++e)
+        acc = fma(-out[static_cast<long>(ci[e]) * K + r], static_cast<double>(val[e]), acc);
+    out[static_cast<long>(t.row) * K + r] = UPPER ? acc / static_cast<double>(t.dv) : acc;
+}
 
 template <int K, typename T, bool UPPER>
 __global__ __launch_bounds__(kSThreads) void k_ssweep(const int* __restrict__ sys_grp, const int* __restrict__ grp_off,
@@ -55,6 +108,7 @@ __global__ __launch_bounds__(kSThreads) void k_ssweep(const int* __restrict__ sy
                                                       const int* __restrict__ ci, const int* __restrict__ dg,
                                                       const T* __restrict__ val, const double* b, double* out,
                                                       const BCtl* ctl) {
+    constexpr int kRowsPerPass = kSThreads / K;
     const int s = blockIdx.x;
     if (ctl) {
         bool any = false;
@@ -63,16 +117,34 @@ __global__ __launch_bounds__(kSThreads) void k_ssweep(const int* __restrict__ sy
         if (!any) return;
     }
     const int r = threadIdx.x % K;
-    for (int g = sys_grp[s]; g < sys_grp[s + 1]; ++g) {
-        const int p1 = grp_off[g + 1];
-        for (int p = grp_off[g] + static_cast<int>(threadIdx.x) / K; p < p1; p += kSThreads / K) {
-            const int row = perm[p];
-            const int d = dg[row];
-            const int end = UPPER ? rp[row + 1] : d;
-            double acc = b[static_cast<long>(row) * K + r];
-            for (int e = UPPER ? d + 1 : rp[row]; e < end; ++e)
-                acc = fma(-out[static_cast<long>(ci[e]) * K + r], static_cast<double>(val[e]), acc);
-            out[static_cast<long>(row) * K + r] = UPPER ? acc / static_cast<double>(val[d]) : acc;
+    const int slot = static_cast<int>(threadIdx.x) / K;
+    const int g0 = sys_grp[s], g1 = sys_grp[s + 1];
+    SsRow<T> nxt[kSsSlots];
+#pragma unroll
+    for (int k = 0; k < kSsSlots; ++k) nxt[k].row = -1;
+    if (g0 < g1) {
+        const int q0 = grp_off[g0], q1 = grp_off[g0 + 1];
+#pragma unroll
+        for (int k = 0; k < kSsSlots; ++k)
+            ss_fetch<K, T, UPPER>(q0 + slot + k * kRowsPerPass, q1, r, perm, rp, ci, dg, val, b, nxt[k]);
+    }
+    for (int g = g0; g < g1; ++g) {
+        const int p0 = grp_off[g], p1 = grp_off[g + 1];
+        SsRow<T> cur[kSsSlots];
+#pragma unroll
+        for (int k = 0; k < kSsSlots; ++k) cur[k] = nxt[k];
+        if (g + 1 < g1) {
+            const int n1 = grp_off[g + 2];
+#pragma unroll
+            for (int k = 0; k < kSsSlots; ++k)
+                ss_fetch<K, T, UPPER>(p1 + slot + k * kRowsPerPass, n1, r, perm, rp, ci, dg, val, b, nxt[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kSsSlots; ++k) ss_solve<K, T, UPPER>(cur[k], r, ci, val, out);
+        for (int p = p0 + slot + kSsSlots * kRowsPerPass; p < p1; p += kRowsPerPass) {
+            SsRow<T> t;
+            ss_fetch<K, T, UPPER>(p, p1, r, perm, rp, ci, dg, val, b, t);
+            ss_solve<K, T, UPPER>(t, r, ci, val, out);
         }
         __syncthreads();
     }
