@@ -28,7 +28,11 @@ namespace pilu {
 namespace {
 
 constexpr int kSymWaves = 4;           // waves (rows) per workgroup
-constexpr int kSymCap = 2048;          // LDS entries per row buffer
+// LDS entries per row buffer: a short-row form (more rows resident, so a
+// longer stretch of the row order is in flight) and the general form, used
+// when a row of F outgrows the short one.
+constexpr int kSymCapShort = 64;
+constexpr int kSymCapLong = 2048;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint64_t kNoHeader = ~0ull;
 constexpr int kLevBits = 5;
@@ -53,12 +57,13 @@ __device__ __forceinline__ int lower_col(const uint32_t* a, int lo, int hi, uint
     return lo;
 }
 
-// err bits: 1 repeated column in a row of A, 2 row over kSymCap, 4 pool full.
+// err bits: 1 repeated column in a row of A, 2 row over CAP, 4 pool full.
+template <int CAP>
 __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
     int n, const int* __restrict__ arp, const int* __restrict__ aci, int lof, uint32_t* pool, long pool_cap,
     unsigned long long* pool_top, uint64_t* header, long* row_off, int* row_cnt, int* err, int* ctr,
     uint64_t wd_limit) {
-    __shared__ uint32_t s_buf[kSymWaves][2][kSymCap];
+    __shared__ uint32_t s_buf[kSymWaves][2][CAP];
     __shared__ uint32_t s_new[kSymWaves][64];
     const int ticket = take_ticket(ctr);
     const Watchdog wd(wd_limit);
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
         uint32_t* fresh = s_new[wave];
         const int ab = arp[i];
         const int r0 = arp[i + 1] - ab;
-        bool failed = r0 > kSymCap;
+        bool failed = r0 > CAP;
         int r = 0;
         if (!failed) {
             // Row i of A at level 0, sorted by column (src/iluk.cpp:11-13).
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
                 const uint64_t m = __ballot(is_new);
                 const int nnew = __popcll(m);
                 if (nnew == 0) continue;
-                if (r + nnew > kSymCap) {
+                if (r + nnew > CAP) {
                     failed = true;
                     if (lane == 0) atomicOr(err, 2);
                     break;
@@ -282,6 +287,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     DevBuf<unsigned long long> top(1);
     long cap = std::max<long>(static_cast<long>(nnz_a) * (lof == 0 ? 1 : 32), 1024);
     DevBuf<uint32_t> pool;
+    bool short_rows = true;  // first try the short-row form; a row over its capacity retries the long form
     for (int attempt = 0;; ++attempt) {
         pool.resize(cap);
         check(hipMemsetAsync(pool.get(), 0xFF, sizeof(uint32_t) * cap, s), "pool fill");
@@ -289,16 +295,25 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
         check(hipMemsetAsync(misc.get(), 0, 8 * sizeof(int), s), "misc");
         check(hipMemsetAsync(top.get(), 0, sizeof(unsigned long long), s), "top");
         const int grid = ceil_div(n, kSymWaves);
-        k_symbolic<<<grid, kSymWaves * 64, 0, s>>>(n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(),
-                                                   row_off.get(), row_cnt.get(), misc.get() + 2, misc.get(),
-                                                   watchdog_ticks(20.0));
+        if (short_rows)
+            k_symbolic<kSymCapShort><<<grid, kSymWaves * 64, 0, s>>>(
+                n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
+                misc.get() + 2, misc.get(), watchdog_ticks(20.0));
+        else
+            k_symbolic<kSymCapLong><<<grid, kSymWaves * 64, 0, s>>>(
+                n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
+                misc.get() + 2, misc.get(), watchdog_ticks(20.0));
         check(hipGetLastError(), "k_symbolic");
         int err = 0;
         check(hipMemcpyAsync(&err, misc.get() + 2, sizeof(int), hipMemcpyDeviceToHost, s), "err");
         check(hipStreamSynchronize(s), "sync");
         if (err & kErrTimeout) throw std::runtime_error("IlukSymbolic: dependency wait timed out (watchdog)");
         if (err & 1) return -1;  // repeated column: the reference exits here
-        if (err & 2) throw std::runtime_error("IlukSymbolic: a row of F exceeds 2048 entries");
+        if (err & 2) {
+            if (!short_rows) throw std::runtime_error("IlukSymbolic: a row of F exceeds 2048 entries");
+            short_rows = false;
+            continue;
+        }
         if (err & 4) {
             if (attempt >= 6) throw std::runtime_error("IlukSymbolic: fill pool exhausted");
             cap *= 4;
