@@ -173,12 +173,15 @@ __global__ __launch_bounds__(kMgsThreads) void k_mgs_all(int n, int i, int m, in
     bool timed_out = false;
     const long r0 = static_cast<long>(wg) * chunk;
     const long r1 = r0 + chunk < n ? r0 + chunk : n;
-    double wr[R], vn[R];
+    // vn holds v_j and vq v_{j+1} at term j; v_{j+2} is requested into vn
+    // before the wait for h_{j+1}, so the load overlaps the reduction.
+    double wr[R], vn[R], vq[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const long r = r0 + threadIdx.x + static_cast<long>(k) * kMgsThreads;
         wr[k] = r < r1 ? w[r] : 0.0;
         vn[k] = r < r1 ? vv[r] : 0.0;
+        vq[k] = r < r1 && i > 0 ? vv[static_cast<long>(n) + r] : 0.0;
     }
     double h = 0.0;
     {
@@ -188,18 +191,30 @@ __global__ __launch_bounds__(kMgsThreads) void k_mgs_all(int n, int i, int m, in
     }
     for (int j = 0; j <= i; ++j) {
         if (wg == 0 && threadIdx.x == 0) col[j] = h;
-        const double* vnext = j < i ? vv + static_cast<long>(j + 1) * n : nullptr;
+        const bool more = j < i;
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const long r = r0 + threadIdx.x + static_cast<long>(k) * kMgsThreads;
             wr[k] = fma(-h, vn[k], wr[k]);
-            if (vnext) vn[k] = r < r1 ? vnext[r] : 0.0;
-            acc = vnext ? fma(vn[k], wr[k], acc) : fma(wr[k], wr[k], acc);
+            acc = more ? fma(vq[k], wr[k], acc) : fma(wr[k], wr[k], acc);
         }
         const double tot = block_sum512(acc);
         double* slot = pb + static_cast<long>(j) * nwg;
         if (threadIdx.x == 0) publish(slot + wg, tot);
+        if (j + 1 < i) {
+            const double* v2 = vv + static_cast<long>(j + 2) * n;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const long r = r0 + threadIdx.x + static_cast<long>(k) * kMgsThreads;
+                vn[k] = r < r1 ? v2[r] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double t = vn[k];
+            vn[k] = vq[k];
+            vq[k] = t;
+        }
         double v = 0.0;
         if (threadIdx.x < nwg) {
             uint64_t bits;

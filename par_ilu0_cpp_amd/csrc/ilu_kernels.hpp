@@ -106,10 +106,45 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
     }
     int lv = 0;
     bool done = !active;
+    // One lane per row: the row's first kLevE dependencies (columns and
+    // block test) are read once and their levels polled together, so a
+    // chain of dependent rows costs one round trip per row instead of two
+    // per dependency; further entries go through the loop below.
+    constexpr int kLevE = G == 1 ? 8 : 1;
+    int hc[kLevE];
+    bool head = G == 1;
+    if constexpr (G == 1) {
+#pragma unroll
+        for (int q = 0; q < kLevE; ++q) {
+            hc[q] = -1;
+            if (active && e + q < end) {
+                const int c = ci[e + q];
+                hc[q] = block_of && block_of[c] != my_block ? -1 : c;
+            }
+        }
+        e = e + kLevE < end ? e + kLevE : end;
+    }
     while (true) {
         if (!done) {
+            if constexpr (G == 1) {
+                if (head) {
+                    bool ok = true;
+#pragma unroll
+                    for (int q = 0; q < kLevE; ++q)
+                        if (hc[q] >= 0) {
+                            const int d = peek_int(level + hc[q]);
+                            if (d < 0) {
+                                ok = false;
+                            } else {
+                                lv = d + 1 > lv ? d + 1 : lv;
+                                hc[q] = -1;
+                            }
+                        }
+                    head = !ok;
+                }
+            }
             // lane t reads entries t, t+G, ...; a pending level stops it until the next round
-            while (e < end) {
+            while (!head && e < end) {
                 const int c = ci[e];
                 if (block_of && block_of[c] != my_block) { e += G; continue; }  // outside the row's block
                 const int d = peek_int(level + c);
@@ -117,11 +152,12 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
                 lv = d + 1 > lv ? d + 1 : lv;
                 e += G;
             }
-            if (e < end && wd.expired()) {
+            if ((head || e < end) && wd.expired()) {
                 atomicOr(err, kErrTimeout);
                 e = end;
+                head = false;
             }
-            if (group_all<G>(e >= end)) {
+            if (group_all<G>(!head && e >= end)) {
                 const int l = group_max<G>(lv);
                 if (t == 0) {
                     publish_int(level + row, l);
