@@ -19,7 +19,7 @@
 
 namespace pilu {
 
-constexpr int kVecBlocks = 1024;  // fixed grid of the reducing vector kernels
+constexpr int kVecBlocks = 2048;  // fixed grid of the reducing vector kernels (8 waves per CU)
 constexpr int kVecThreads = 256;
 
 struct GmresCtl {
