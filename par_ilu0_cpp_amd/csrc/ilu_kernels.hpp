@@ -105,6 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
         end = UPPER ? rp[row + 1] : dg[row];
     }
     int lv = 0;
+    int top = -1;  // highest level published by this lane
     bool done = !active;
     // One lane per row: the row's first kLevE dependencies (columns and
     // block test) are read once and their levels polled together, so a
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
                 const int l = group_max<G>(lv);
                 if (t == 0) {
                     publish_int(level + row, l);
-                    atomicMax(maxlev, l);
+                    top = l > top ? l : top;
                 }
                 done = true;
             }
@@ -169,6 +170,14 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
         if (__ballot(!done) == 0) break;
         spin_pause();
     }
+    // one atomic per wave for the level count (one per row serialises on
+    // the single counter word)
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_xor(top, off, 64);
+        top = o > top ? o : top;
+    }
+    if (lane == 0 && top >= 0) atomicMax(maxlev, top);
     retire_ticket(ctr);
 }
 

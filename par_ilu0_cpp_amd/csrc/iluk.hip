@@ -191,9 +191,14 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
         // Publish the row (src/iluk.cpp:29,34-47 pattern part).
         long off = 0;
         if (!failed) {
-            unsigned long long o = 0;
-            if (lane == 0) o = atomicAdd(pool_top, static_cast<unsigned long long>(r));
-            off = static_cast<long>(__shfl(static_cast<long long>(o), 0));
+            if constexpr (CAP == kSymCapShort) {
+                // short rows: a fixed slot of CAP entries per row, no shared counter
+                off = static_cast<long>(i) * CAP;
+            } else {
+                unsigned long long o = 0;
+                if (lane == 0) o = atomicAdd(pool_top, static_cast<unsigned long long>(r));
+                off = static_cast<long>(__shfl(static_cast<long long>(o), 0));
+            }
             if (off + r > pool_cap) {
                 failed = true;
                 if (lane == 0) atomicOr(err, 4);
@@ -289,6 +294,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     DevBuf<uint32_t> pool;
     bool short_rows = true;  // first try the short-row form; a row over its capacity retries the long form
     for (int attempt = 0;; ++attempt) {
+        if (short_rows) cap = std::max<long>(cap, static_cast<long>(n) * kSymCapShort);
         pool.resize(cap);
         check(hipMemsetAsync(pool.get(), 0xFF, sizeof(uint32_t) * cap, s), "pool fill");
         check(hipMemsetAsync(header.get(), 0xFF, sizeof(uint64_t) * n, s), "header fill");
