@@ -165,6 +165,7 @@ def main():
         nl, nu = ilu.Levels()
         log(f"[bench] {form}: symbolic+analysis {t_setup*1e3:.1f} ms nnz(F)={nnz_f} levels L={nl} U={nu}")
         sweep_cfg = ilu.SweepConfig()
+        ts = time.perf_counter()
         ilu.SetGridShape(s, s, s)  # the workload's grid: lets the sweeps use 3D tiles
         m = 2 if bits == 64 else 3
         spec = args.sweep or f"{m}:8:{m}:8:{min(s, 128)}x4x8"
@@ -176,7 +177,10 @@ def main():
             elif int(ml) or int(mu):
                 ilu.SetSweepMode(int(ml), int(mu), int(blk))
             sweep_cfg = ilu.SweepConfig()
-            log(f"[bench] {form}: fixed sweep shape {sweep_cfg}")
+            torch.cuda.synchronize()
+            t_part = time.perf_counter() - ts
+            t_setup += t_part
+            log(f"[bench] {form}: fixed sweep shape {sweep_cfg} (partition and packed pattern {t_part*1e3:.1f} ms)")
         elif spec == "auto":
             # one factorization, then time every sweep shape (part of setup, untimed)
             if ilu.FactorizeDevice(d_va.data_ptr(), True) != 0:

@@ -10,6 +10,9 @@
 #include <cstring>
 #include <string>
 #include <stdexcept>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "ilu_internal.hpp"
@@ -61,6 +64,48 @@ void check_sweep_errors(lljbash::IluSolver::Ext* e) {
 }
 
 namespace {
+
+// f(begin, end) over [0, n) in contiguous chunks on up to 16 host threads.
+template <typename F>
+void parallel_for(long n, F f) {
+    const long hw = static_cast<long>(std::thread::hardware_concurrency());
+    const long t = std::max(1L, std::min({hw > 0 ? hw : 1L, 16L, n / 65536 + 1}));
+    if (t == 1) {
+        f(0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(static_cast<size_t>(t));
+    for (long k = 0; k < t; ++k) th.emplace_back(f, n * k / t, n * (k + 1) / t);
+    for (auto& x : th) x.join();
+}
+
+// Host copy of the pattern (row_ptr and diagonal positions are kept by
+// analyse; the columns are copied on first use).
+void host_pattern(lljbash::IluSolver::Ext* e) {
+    if (static_cast<long>(e->h_ci.size()) != e->nnz) {
+        e->h_ci.resize(static_cast<size_t>(e->nnz));
+        PILU_HIP(hipMemcpy(e->h_ci.data(), e->ci.get(), sizeof(int) * e->h_ci.size(), hipMemcpyDeviceToHost));
+    }
+}
+
+// Grid coordinates of row q: q = x + nx (y + ny z), exact for q < 2^31 (a
+// reciprocal estimate, corrected by one step).
+struct GridCoord {
+    long nx, nxy;
+    double inx, inxy;
+    GridCoord(long nx_, long ny_) : nx(nx_), nxy(nx_ * ny_), inx(1.0 / nx_), inxy(1.0 / (nx_ * ny_)) {}
+    void operator()(long q, long& x, long& y, long& z) const {
+        z = static_cast<long>(static_cast<double>(q) * inxy);
+        if (z * nxy > q) --z;
+        else if ((z + 1) * nxy <= q) ++z;
+        const long rem = q - z * nxy;
+        y = static_cast<long>(static_cast<double>(rem) * inx);
+        if (y * nx > rem) --y;
+        else if ((y + 1) * nx <= rem) ++y;
+        x = rem - y * nx;
+    }
+};
 
 using Ext = lljbash::IluSolver::Ext;
 
@@ -208,20 +253,28 @@ std::vector<int> block_order(const Ext* e, const std::vector<int>& rp, const std
     for (int b = 0; b < nblk; ++b) start[b + 1] += start[b];
     std::vector<int> fill(start.begin(), start.end() - 1);
     for (int i = 0; i < n; ++i) rows[fill[bo[i]]++] = i;
-    // predecessor lists, deduplicated with a marker per block
-    std::vector<int> mark(nblk, -1), pred_ptr(static_cast<size_t>(nblk) + 1, 0), pred;
-    for (int b = 0; b < nblk; ++b) {
-        for (int k = start[b]; k < start[b + 1]; ++k) {
-            const int i = rows[k];
-            const int e0 = upper ? dg[i] + 1 : rp[i], e1 = upper ? rp[i + 1] : dg[i];
-            for (int p = e0; p < e1; ++p) {
-                const int bj = bo[ci[p]];
-                if (bj != b && mark[bj] != b) {
-                    mark[bj] = b;
-                    pred.push_back(bj);
+    // predecessor lists, deduplicated with a marker per block (blocks in
+    // parallel, each thread with its own marker array)
+    std::vector<std::vector<int>> pl(static_cast<size_t>(nblk));
+    parallel_for(nblk, [&](long lo, long hi) {
+        std::vector<int> mark(nblk, -1);
+        for (long b = lo; b < hi; ++b) {
+            for (int k = start[b]; k < start[b + 1]; ++k) {
+                const int i = rows[k];
+                const int e0 = upper ? dg[i] + 1 : rp[i], e1 = upper ? rp[i + 1] : dg[i];
+                for (int p = e0; p < e1; ++p) {
+                    const int bj = bo[ci[p]];
+                    if (bj != b && mark[bj] != b) {
+                        mark[bj] = static_cast<int>(b);
+                        pl[b].push_back(bj);
+                    }
                 }
             }
         }
+    });
+    std::vector<int> pred_ptr(static_cast<size_t>(nblk) + 1, 0), pred;
+    for (int b = 0; b < nblk; ++b) {
+        pred.insert(pred.end(), pl[b].begin(), pl[b].end());
         pred_ptr[b + 1] = static_cast<int>(pred.size());
     }
     // successor lists and Kahn's algorithm, computing depths on the way
@@ -277,12 +330,9 @@ void install_partition(Ext* e, int nblk, bool contiguous) {
             pos_u[b] = nblk - 1 - b;
         }
     } else {
-        std::vector<int> rp(static_cast<size_t>(n) + 1), dg(n), ci(static_cast<size_t>(e->nnz));
-        PILU_HIP(hipMemcpy(rp.data(), e->rp.get(), sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
-        PILU_HIP(hipMemcpy(dg.data(), e->dg.get(), sizeof(int) * dg.size(), hipMemcpyDeviceToHost));
-        PILU_HIP(hipMemcpy(ci.data(), e->ci.get(), sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
-        pos_l = block_order(e, rp, ci, dg, false);
-        pos_u = block_order(e, rp, ci, dg, true);
+        host_pattern(e);
+        pos_l = block_order(e, e->h_rp, e->h_ci, e->h_dg, false);
+        pos_u = block_order(e, e->h_rp, e->h_ci, e->h_dg, true);
     }
     build_block_schedule(e, false, e->block_of.get(), e->nblk, pos_l, e->bperm_l, e->blev_l, e->loff_l);
     build_block_schedule(e, true, e->block_of.get(), e->nblk, pos_u, e->bperm_u, e->blev_u, e->loff_u);
@@ -317,39 +367,73 @@ void use_tiles(Ext* e, int tx, int ty, int tz) {
         throw std::runtime_error("tile must hold 1..16384 rows");
     if (e->tile[0] == tx && e->tile[1] == ty && e->tile[2] == tz) return;
     const int n = e->n;
-    std::vector<int> rp(static_cast<size_t>(n) + 1), dg(n), ci(static_cast<size_t>(e->nnz));
-    PILU_HIP(hipMemcpy(rp.data(), e->rp.get(), sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
-    PILU_HIP(hipMemcpy(dg.data(), e->dg.get(), sizeof(int) * dg.size(), hipMemcpyDeviceToHost));
-    PILU_HIP(hipMemcpy(ci.data(), e->ci.get(), sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
-    const long nxy = static_cast<long>(nx) * ny;
+    host_pattern(e);
+    const std::vector<int>& rp = e->h_rp;
+    const std::vector<int>& dg = e->h_dg;
+    const std::vector<int>& ci = e->h_ci;
+    const GridCoord gc(nx, ny);
     auto ceil_div_signed = [](long a, long b) { return a >= 0 ? (a + b - 1) / b : -((-a) / b); };
     long b = 0, c = 0, c2 = 0;
     for (int pass = 0; pass < 2; ++pass) {  // b first, then c (which depends on b)
-        for (int r = 0; r < n; ++r) {
-            const long x = r % nx, y = (r / nx) % ny, z = r / nxy;
-            for (int p = rp[r]; p < dg[r]; ++p) {
-                const long q = ci[p];
-                const long dx = q % nx - x, dy = (q / nx) % ny - y, dz = q / nxy - z;
-                if (pass == 0) {
-                    if (dz == 0 && dy < 0) b = std::max(b, ceil_div_signed(dx, -dy));
-                    if (dz < 0) c2 = std::max(c2, ceil_div_signed(dy, -dz));
-                } else if (dz < 0) {
-                    c = std::max(c, ceil_div_signed(dx + b * dy, -dz));
+        std::mutex mu;
+        parallel_for(n, [&](long lo, long hi) {
+            long lb = 0, lc = 0, lc2 = 0;
+            long x, y, z;
+            gc(lo, x, y, z);
+            for (long r = lo; r < hi; ++r) {
+                for (int p = rp[r]; p < dg[r]; ++p) {
+                    long qx, qy, qz;
+                    gc(ci[p], qx, qy, qz);
+                    const long dx = qx - x, dy = qy - y, dz = qz - z;
+                    if (pass == 0) {
+                        if (dz == 0 && dy < 0) lb = std::max(lb, ceil_div_signed(dx, -dy));
+                        if (dz < 0) lc2 = std::max(lc2, ceil_div_signed(dy, -dz));
+                    } else if (dz < 0) {
+                        lc = std::max(lc, ceil_div_signed(dx + b * dy, -dz));
+                    }
+                }
+                if (++x == nx) {
+                    x = 0;
+                    if (++y == ny) {
+                        y = 0;
+                        ++z;
+                    }
                 }
             }
-        }
+            std::lock_guard<std::mutex> g(mu);
+            if (pass == 0) {
+                b = std::max(b, lb);
+                c2 = std::max(c2, lc2);
+            } else {
+                c = std::max(c, lc);
+            }
+        });
     }
     const long umax = (nx - 1) + b * (ny - 1) + c * (nz - 1), vmax = (ny - 1) + c2 * (nz - 1);
     const long ntu = umax / tx + 1, ntv = vmax / ty + 1, ntw = (nz - 1) / tz + 1;
     if (ntu * ntv * ntw > (1L << 30)) throw std::runtime_error("tiled sweeps: too many tiles");
-    // dense ids of the non-empty tiles, in (w, v, u) order
+    // dense ids of the non-empty tiles, in order of their first row
+    std::vector<long> tile_of(static_cast<size_t>(n));
+    parallel_for(n, [&](long lo, long hi) {
+        long x, y, z;
+        gc(lo, x, y, z);
+        for (long r = lo; r < hi; ++r) {
+            const long u = x + b * y + c * z, v = y + c2 * z;
+            tile_of[r] = ((z / tz) * ntv + v / ty) * ntu + u / tx;
+            if (++x == nx) {
+                x = 0;
+                if (++y == ny) {
+                    y = 0;
+                    ++z;
+                }
+            }
+        }
+    });
     std::vector<int> id(static_cast<size_t>(ntu * ntv * ntw), -1);
     e->h_block_of.resize(n);
     int nblk = 0;
     for (int r = 0; r < n; ++r) {
-        const long x = r % nx, y = (r / nx) % ny, z = r / nxy;
-        const long u = x + b * y + c * z, v = y + c2 * z;
-        const long t = ((z / tz) * ntv + v / ty) * ntu + u / tx;
+        const long t = tile_of[r];
         if (id[t] < 0) id[t] = nblk++;
         e->h_block_of[r] = id[t];
     }
@@ -391,17 +475,18 @@ void analyse(Ext* e, const int* host_rp) {
 
     int max_row = 0, max_lo = 0, max_up = 0;
     {
-        std::vector<int> h, hd(n);
-        if (!host_rp) {
-            h.resize(static_cast<size_t>(n) + 1);
-            PILU_HIP(hipMemcpy(h.data(), e->rp.get(), h.size() * sizeof(int), hipMemcpyDeviceToHost));
-        }
-        const int* r = host_rp ? host_rp : h.data();
+        std::vector<int>& h = e->h_rp;
+        std::vector<int>& hd = e->h_dg;
+        h.resize(static_cast<size_t>(n) + 1);
+        hd.resize(static_cast<size_t>(n));
+        e->h_ci.clear();  // columns are copied again on first use
+        if (host_rp) std::copy(host_rp, host_rp + n + 1, h.begin());
+        else PILU_HIP(hipMemcpy(h.data(), e->rp.get(), h.size() * sizeof(int), hipMemcpyDeviceToHost));
         PILU_HIP(hipMemcpy(hd.data(), e->dg.get(), sizeof(int) * static_cast<size_t>(n), hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
-            max_row = std::max(max_row, r[i + 1] - r[i]);
-            max_lo = std::max(max_lo, hd[i] - r[i]);
-            max_up = std::max(max_up, r[i + 1] - hd[i] - 1);
+            max_row = std::max(max_row, h[i + 1] - h[i]);
+            max_lo = std::max(max_lo, hd[i] - h[i]);
+            max_up = std::max(max_up, h[i + 1] - hd[i] - 1);
         }
     }
     e->max_row = max_row;
@@ -434,11 +519,12 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
     const DevBuf<int>& perm = upper ? e->bperm_u : e->bperm_l;
     const DevBuf<int>& blev = upper ? e->blev_u : e->blev_l;
     const DevBuf<int>& loff = upper ? e->loff_u : e->loff_l;
-    std::vector<int> rp(static_cast<size_t>(n) + 1), dg(n), pm(n), ci(static_cast<size_t>(e->nnz));
-    PILU_HIP(hipMemcpy(rp.data(), e->rp.get(), sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
-    PILU_HIP(hipMemcpy(dg.data(), e->dg.get(), sizeof(int) * dg.size(), hipMemcpyDeviceToHost));
+    host_pattern(e);
+    const std::vector<int>& rp = e->h_rp;
+    const std::vector<int>& dg = e->h_dg;
+    const std::vector<int>& ci = e->h_ci;
+    std::vector<int> pm(n);
     PILU_HIP(hipMemcpy(pm.data(), perm.get(), sizeof(int) * pm.size(), hipMemcpyDeviceToHost));
-    PILU_HIP(hipMemcpy(ci.data(), e->ci.get(), sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
     pk.h_blev.assign(static_cast<size_t>(nblk) + 1, 0);
     PILU_HIP(hipMemcpy(pk.h_blev.data(), blev.get(), sizeof(int) * pk.h_blev.size(), hipMemcpyDeviceToHost));
     const int ngroups = pk.h_blev[nblk];
@@ -446,55 +532,74 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
     PILU_HIP(hipMemcpy(pk.h_loff.data(), loff.get(), sizeof(int) * pk.h_loff.size(), hipMemcpyDeviceToHost));
     // schedule position of every row and the first position of its block
     std::vector<int> kinv(n), kptr(static_cast<size_t>(nblk) + 1), kb0(n);
-    for (int k = 0; k < n; ++k) kinv[pm[k]] = k;
+    parallel_for(n, [&](long lo, long hi) {
+        for (long k = lo; k < hi; ++k) kinv[pm[k]] = static_cast<int>(k);
+    });
     for (int p = 0; p <= nblk; ++p) kptr[p] = pk.h_loff[pk.h_blev[p]];
-    for (int p = 0; p < nblk; ++p)
-        for (int k = kptr[p]; k < kptr[p + 1]; ++k) kb0[pm[k]] = kptr[p];
+    parallel_for(nblk, [&](long lo, long hi) {
+        for (long p = lo; p < hi; ++p)
+            for (int k = kptr[p]; k < kptr[p + 1]; ++k) kb0[pm[k]] = kptr[p];
+    });
     pk.h_base.assign(ngroups, 0);
     pk.h_width.assign(ngroups, 0);
     std::vector<int> rowinfo(2 * static_cast<size_t>(n));
+    parallel_for(ngroups, [&](long lo, long hi) {
+        for (long g = lo; g < hi; ++g) {
+            int w = 0;
+            for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
+                const int row = pm[k];
+                const int cnt = upper ? rp[row + 1] - dg[row] - 1 : dg[row] - rp[row];
+                rowinfo[2 * static_cast<size_t>(k)] = row;
+                rowinfo[2 * static_cast<size_t>(k) + 1] = cnt;
+                w = std::max(w, cnt);
+            }
+            pk.h_width[g] = w;
+        }
+    });
     long total = 0;
     int maxw = 0;
     for (int g = 0; g < ngroups; ++g) {
-        int w = 0;
-        for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
-            const int row = pm[k];
-            const int cnt = upper ? rp[row + 1] - dg[row] - 1 : dg[row] - rp[row];
-            rowinfo[2 * static_cast<size_t>(k)] = row;
-            rowinfo[2 * static_cast<size_t>(k) + 1] = cnt;
-            w = std::max(w, cnt);
-        }
         pk.h_base[g] = static_cast<int>(total);
-        pk.h_width[g] = w;
-        maxw = std::max(maxw, w);
-        total += static_cast<long>(w) * (pk.h_loff[g + 1] - pk.h_loff[g]);
+        maxw = std::max(maxw, pk.h_width[g]);
+        total += static_cast<long>(pk.h_width[g]) * (pk.h_loff[g + 1] - pk.h_loff[g]);
         if (total >= (1L << 31)) throw std::runtime_error("packed sweep: too many entries");
     }
     const std::vector<int>& bo = e->h_block_of;
-    std::vector<int> pcol(std::max<long>(total, 1), 0), psrc(std::max<long>(total, 1), -1);
-    for (int g = 0; g < ngroups; ++g) {
-        const int count = pk.h_loff[g + 1] - pk.h_loff[g];
-        for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
-            const int row = pm[k];
-            const int cnt = rowinfo[2 * static_cast<size_t>(k) + 1];
-            const int e0 = upper ? dg[row] + 1 : rp[row];
-            for (int q = 0; q < cnt; ++q) {
-                const long pos = pk.h_base[g] + static_cast<long>(q) * count + (k - pk.h_loff[g]);
-                const int c = ci[e0 + q];
-                pcol[pos] = bo[c] == bo[row] ? -1 - (kinv[c] - kb0[row]) : c;
-                psrc[pos] = e0 + q;
+    // every slab slot is written below (padding: column 0, source -1)
+    const size_t ne_alloc = static_cast<size_t>(std::max<long>(total, 1));
+    std::unique_ptr<int[]> pcol(new int[ne_alloc]), psrc(new int[ne_alloc]);
+    pcol[0] = 0;
+    psrc[0] = -1;
+    parallel_for(ngroups, [&](long lo, long hi) {
+        for (long g = lo; g < hi; ++g) {
+            const int count = pk.h_loff[g + 1] - pk.h_loff[g], width = pk.h_width[g];
+            for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
+                const int row = pm[k];
+                const int cnt = rowinfo[2 * static_cast<size_t>(k) + 1];
+                const int e0 = upper ? dg[row] + 1 : rp[row];
+                for (int q = 0; q < width; ++q) {
+                    const long pos = pk.h_base[g] + static_cast<long>(q) * count + (k - pk.h_loff[g]);
+                    if (q < cnt) {
+                        const int c = ci[e0 + q];
+                        pcol[pos] = bo[c] == bo[row] ? -1 - (kinv[c] - kb0[row]) : c;
+                        psrc[pos] = e0 + q;
+                    } else {
+                        pcol[pos] = 0;
+                        psrc[pos] = -1;
+                    }
+                }
             }
         }
-    }
+    });
     pk.ne = total;
     pk.max_w = maxw;
     pk.rowinfo.resize(rowinfo.size());
-    pk.pcol.resize(pcol.size());
-    pk.psrc.resize(psrc.size());
+    pk.pcol.resize(ne_alloc);
+    pk.psrc.resize(ne_alloc);
     pk.kptr.resize(kptr.size());
     PILU_HIP(hipMemcpy(pk.rowinfo.get(), rowinfo.data(), sizeof(int) * rowinfo.size(), hipMemcpyHostToDevice));
-    PILU_HIP(hipMemcpy(pk.pcol.get(), pcol.data(), sizeof(int) * pcol.size(), hipMemcpyHostToDevice));
-    PILU_HIP(hipMemcpy(pk.psrc.get(), psrc.data(), sizeof(int) * psrc.size(), hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(pk.pcol.get(), pcol.get(), sizeof(int) * ne_alloc, hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(pk.psrc.get(), psrc.get(), sizeof(int) * ne_alloc, hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(pk.kptr.get(), kptr.data(), sizeof(int) * kptr.size(), hipMemcpyHostToDevice));
     pk.version = e->part_version;
     pk.pass_g = 0;
