@@ -228,22 +228,59 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, const int* __restrict__
             if (reset) reinterpret_cast<uint64_t*>(reset)[row] = kPending;
         }
     }
+    // The lane's first kSwH entries (t, t+G, ...): columns and values read
+    // once, their x polled together; later entries one at a time below.
+    // The lane sums them in the same order either way.
+    constexpr int kSwH = 4;
+    int hc[kSwH];
+    T hv[kSwH];
+    uint64_t hb[kSwH];
+    int nh = 0;
+#pragma unroll
+    for (int q = 0; q < kSwH; ++q) {
+        hc[q] = 0;
+        hv[q] = T(0);
+        hb[q] = kPending;
+        if (active && e + q * G < end) {
+            hc[q] = ci[e + q * G];
+            hv[q] = val[e + q * G];
+            nh = q + 1;
+        }
+    }
+    bool head = nh > 0;
+    e = e + kSwH * G < end ? e + kSwH * G : end;
     bool done = !active;
     while (true) {
         if (!done) {
-            while (e < end) {
+            if (head) {
+                bool ok = true;
+#pragma unroll
+                for (int q = 0; q < kSwH; ++q)
+                    if (q < nh) {
+                        if (hb[q] == kPending) hb[q] = peek_bits(out + hc[q]);
+                        ok = ok && hb[q] != kPending;
+                    }
+                if (ok) {
+#pragma unroll
+                    for (int q = 0; q < kSwH; ++q)
+                        if (q < nh) acc = fma(-__longlong_as_double(hb[q]), static_cast<double>(hv[q]), acc);
+                    head = false;
+                }
+            }
+            while (!head && e < end) {
                 const uint64_t bits = peek_bits(out + ci[e]);
                 if (bits == kPending) break;
                 acc = fma(-__longlong_as_double(bits), static_cast<double>(val[e]), acc);
                 e += G;
             }
-            if (e < end && wd.expired()) {
+            if ((head || e < end) && wd.expired()) {
                 atomicOr(err, kErrTimeout);
                 acc = __longlong_as_double(kQuietNan);
                 e = end;
+                head = false;
             }
         }
-        const bool ready = group_all<G>(done || e >= end);
+        const bool ready = group_all<G>(done || (!head && e >= end));
         if (ready && !done) {
             const double s = group_sum<G>(acc);
             if (t == 0) publish(out + row, UPPER ? s / dval : s);
