@@ -399,6 +399,7 @@ __global__ __launch_bounds__(512) void k_sweep_blk(int n, int blk, int nblk, con
 // Passes per unrolled step of k_sweep_pp; every block's descriptor list is
 // padded to a multiple of it with empty passes (cb = 0: no rows).
 constexpr int kPpUnroll = 16;
+constexpr int kPpThreads = 512;  // threads per workgroup of the packed sweep
 
 struct PassDesc {
     int k_lo;   // schedule position of the pass's first row
@@ -483,7 +484,7 @@ __device__ __forceinline__ void pp_issue_x(int t, int idle, const PpA<S>& a, con
 // t+DA (DX < DB < DA).  The first two only depend on row data requested
 // DA-DB or more passes earlier, so no issue waits on a young request.
 template <int G, int S, int DA, int DB, int DX, typename T, bool UPPER, bool FLAGS>
-__global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, const int* __restrict__ pass_ptr,
+__global__ __launch_bounds__(kPpThreads) void k_sweep_pp(const int* __restrict__ kptr, const int* __restrict__ pass_ptr,
                                                   const PassDesc* __restrict__ desc,
                                                   const int2* __restrict__ rowinfo, const int* __restrict__ pcol,
                                                   const T* __restrict__ pval, const T* __restrict__ pdiag,
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
     // b may alias the companion buffer (in-place apply): the barrier form
     // reads each b_i into LDS before re-arming it; the flag form reads b_i
     // through the prefetch ring and re-arms row i once it has used it.
-    for (int i = threadIdx.x; i < nrows; i += 512) {
+    for (int i = threadIdx.x; i < nrows; i += kPpThreads) {
         const int row = rowinfo[kb0 + i].x;
         if (FLAGS) {
             xs[i] = __longlong_as_double(kPending);
@@ -509,14 +510,14 @@ __global__ __launch_bounds__(512) void k_sweep_pp(const int* __restrict__ kptr, 
             if (reset) reset[row] = __longlong_as_double(kPending);
         }
     }
-    for (int i = threadIdx.x; i < np; i += 512) sd[i] = desc[p0 + i];
+    for (int i = threadIdx.x; i < np; i += kPpThreads) sd[i] = desc[p0 + i];
     __syncthreads();
     if (np == 0) {
         retire_ticket(ctr);
         return;
     }
     const int idle = rowinfo[kb0].x;
-    constexpr int per = 512 / G;
+    constexpr int per = kPpThreads / G;
     const int gid = threadIdx.x / G;
     const int t = threadIdx.x & (G - 1);
     bool timed_out = false;

@@ -607,10 +607,10 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
 }
 
 // Pass list of a pack for G lanes per row: every level group in chunks of
-// 512 / G rows, each block's list padded to a multiple of kPpUnroll.
+// kPpThreads / G rows, each block's list padded to a multiple of kPpUnroll.
 void build_passes(Ext* e, SweepPack& pk, int G) {
     if (pk.pass_g == G) return;
-    const int per = 512 / G;
+    const int per = kPpThreads / G;
     std::vector<PassDesc> desc;
     std::vector<int> dptr(static_cast<size_t>(e->nblk) + 1, 0);
     pk.max_np = 0;
@@ -755,7 +755,7 @@ void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double
     auto kern = k_sweep_pp<G, S, kPpUnroll, 4, 2, T, UPPER, FLAGS>;
     PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
-    kern<<<e->nblk, 512, lds, e->stream>>>(pk.kptr.get(), pk.desc_ptr.get(),
+    kern<<<e->nblk, kPpThreads, lds, e->stream>>>(pk.kptr.get(), pk.desc_ptr.get(),
                                           reinterpret_cast<const PassDesc*>(pk.pdesc.get()),
                                           reinterpret_cast<const int2*>(pk.rowinfo.get()), pk.pcol.get(), pval, pdiag,
                                           b, out, reset, gate, ctr, e->err.get() + kErrSweep,
