@@ -426,8 +426,9 @@ def cpu_baseline(a, b, tol, gpu_its, steps):
     release flags), timed on a bounded sample: one factorization, then
     `steps` Arnoldi steps of GMRES(50).  The time to solution is estimated
     as factorization + per-step time x the GPU's iteration count.  Run with
-    every thread the process may use and with one thread (the reference
-    benchmark's default -t 1)."""
+    every thread the process may use, with 8 (the reference caps its sweeps
+    at 8 threads) and with one thread (the reference benchmark's default
+    -t 1)."""
     from oracle import py_oracle as O
 
     t0 = time.perf_counter()
@@ -435,7 +436,7 @@ def cpu_baseline(a, b, tol, gpu_its, steps):
     t_sym = time.perf_counter() - t0
     threads = O.baseline_threads()
     runs = {}
-    for th in sorted({threads, 1}, reverse=True):
+    for th in sorted({threads, min(threads, 8), 1}, reverse=True):
         O.baseline_threads(th)
         ret, its, x, times = O.baseline_ilu_gmres((a.row_ptr, a.col_idx, a.value), (frp, fci, fval), b, tol, 50,
                                                   steps)
