@@ -292,7 +292,10 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     DevBuf<unsigned long long> top(1);
     long cap = std::max<long>(static_cast<long>(nnz_a) * (lof == 0 ? 1 : 32), 1024);
     DevBuf<uint32_t> pool;
-    bool short_rows = true;  // first try the short-row form; a row over its capacity retries the long form
+    // first try the short-row form (fixed pool slots, n x 64 entries: only
+    // while that stays within 16 GiB); a row over its capacity retries the
+    // long form
+    bool short_rows = static_cast<long>(n) * kSymCapShort <= (1L << 32);
     for (int attempt = 0;; ++attempt) {
         if (short_rows) cap = std::max<long>(cap, static_cast<long>(n) * kSymCapShort);
         pool.resize(cap);
