@@ -514,15 +514,63 @@ void analyse(Ext* e, const int* host_rp) {
 // of every (block, level) group, in-block columns encoded as
 // -(1 + schedule position within the block), and kptr (first schedule
 // position of every processed block).
+// Device side of the pack build: inverse schedule, first position of each
+// row's block, group of each position, and the slab fill.
+__global__ void k_pack_inv(int n, const int* __restrict__ perm, int* __restrict__ kinv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) kinv[perm[k]] = k;
+}
+
+__global__ void k_pack_block_start(int nblk, const int* __restrict__ kptr, const int* __restrict__ perm,
+                                   int* __restrict__ kb0) {
+    const int p = blockIdx.x;
+    if (p >= nblk) return;
+    for (int k = kptr[p] + threadIdx.x; k < kptr[p + 1]; k += blockDim.x) kb0[perm[k]] = kptr[p];
+}
+
+__global__ void k_pack_group_of(int ngroups, const int* __restrict__ loff, int* __restrict__ grp_of) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroups)
+        for (int k = loff[g]; k < loff[g + 1]; ++k) grp_of[k] = g;
+}
+
+// Slab of every schedule position k: entry q of the row sits at
+// base[g] + q * count + (k - loff[g]); an in-block column is stored as
+// -(1 + its position within the block), padding as column 0, source -1.
+template <bool UPPER>
+__global__ void k_pack_fill(int n, const int* __restrict__ perm, const int* __restrict__ grp_of,
+                            const int* __restrict__ loff, const int* __restrict__ base,
+                            const int* __restrict__ width, const int* __restrict__ rp, const int* __restrict__ dg,
+                            const int* __restrict__ ci, const int* __restrict__ block_of,
+                            const int* __restrict__ kinv, const int* __restrict__ kb0, int* __restrict__ pcol,
+                            int* __restrict__ psrc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int g = grp_of[k], lo = loff[g], count = loff[g + 1] - lo, w = width[g];
+    const int row = perm[k];
+    const int e0 = UPPER ? dg[row] + 1 : rp[row];
+    const int cnt = UPPER ? rp[row + 1] - dg[row] - 1 : dg[row] - rp[row];
+    const int brow = block_of[row], k0 = kb0[row];
+    for (int q = 0; q < w; ++q) {
+        const long pos = base[g] + static_cast<long>(q) * count + (k - lo);
+        if (q < cnt) {
+            const int c = ci[e0 + q];
+            pcol[pos] = block_of[c] == brow ? -1 - (kinv[c] - k0) : c;
+            psrc[pos] = e0 + q;
+        } else {
+            pcol[pos] = 0;
+            psrc[pos] = -1;
+        }
+    }
+}
+
 void build_pack(Ext* e, bool upper, SweepPack& pk) {
     const int n = e->n, nblk = e->nblk;
     const DevBuf<int>& perm = upper ? e->bperm_u : e->bperm_l;
     const DevBuf<int>& blev = upper ? e->blev_u : e->blev_l;
     const DevBuf<int>& loff = upper ? e->loff_u : e->loff_l;
-    host_pattern(e);
     const std::vector<int>& rp = e->h_rp;
     const std::vector<int>& dg = e->h_dg;
-    const std::vector<int>& ci = e->h_ci;
     std::vector<int> pm(n);
     PILU_HIP(hipMemcpy(pm.data(), perm.get(), sizeof(int) * pm.size(), hipMemcpyDeviceToHost));
     pk.h_blev.assign(static_cast<size_t>(nblk) + 1, 0);
@@ -530,16 +578,8 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
     const int ngroups = pk.h_blev[nblk];
     pk.h_loff.assign(static_cast<size_t>(ngroups) + 1, 0);
     PILU_HIP(hipMemcpy(pk.h_loff.data(), loff.get(), sizeof(int) * pk.h_loff.size(), hipMemcpyDeviceToHost));
-    // schedule position of every row and the first position of its block
-    std::vector<int> kinv(n), kptr(static_cast<size_t>(nblk) + 1), kb0(n);
-    parallel_for(n, [&](long lo, long hi) {
-        for (long k = lo; k < hi; ++k) kinv[pm[k]] = static_cast<int>(k);
-    });
+    std::vector<int> kptr(static_cast<size_t>(nblk) + 1);
     for (int p = 0; p <= nblk; ++p) kptr[p] = pk.h_loff[pk.h_blev[p]];
-    parallel_for(nblk, [&](long lo, long hi) {
-        for (long p = lo; p < hi; ++p)
-            for (int k = kptr[p]; k < kptr[p + 1]; ++k) kb0[pm[k]] = kptr[p];
-    });
     pk.h_base.assign(ngroups, 0);
     pk.h_width.assign(ngroups, 0);
     std::vector<int> rowinfo(2 * static_cast<size_t>(n));
@@ -564,43 +604,44 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
         total += static_cast<long>(pk.h_width[g]) * (pk.h_loff[g + 1] - pk.h_loff[g]);
         if (total >= (1L << 31)) throw std::runtime_error("packed sweep: too many entries");
     }
-    const std::vector<int>& bo = e->h_block_of;
-    // every slab slot is written below (padding: column 0, source -1)
-    const size_t ne_alloc = static_cast<size_t>(std::max<long>(total, 1));
-    std::unique_ptr<int[]> pcol(new int[ne_alloc]), psrc(new int[ne_alloc]);
-    pcol[0] = 0;
-    psrc[0] = -1;
-    parallel_for(ngroups, [&](long lo, long hi) {
-        for (long g = lo; g < hi; ++g) {
-            const int count = pk.h_loff[g + 1] - pk.h_loff[g], width = pk.h_width[g];
-            for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
-                const int row = pm[k];
-                const int cnt = rowinfo[2 * static_cast<size_t>(k) + 1];
-                const int e0 = upper ? dg[row] + 1 : rp[row];
-                for (int q = 0; q < width; ++q) {
-                    const long pos = pk.h_base[g] + static_cast<long>(q) * count + (k - pk.h_loff[g]);
-                    if (q < cnt) {
-                        const int c = ci[e0 + q];
-                        pcol[pos] = bo[c] == bo[row] ? -1 - (kinv[c] - kb0[row]) : c;
-                        psrc[pos] = e0 + q;
-                    } else {
-                        pcol[pos] = 0;
-                        psrc[pos] = -1;
-                    }
-                }
-            }
-        }
-    });
     pk.ne = total;
     pk.max_w = maxw;
     pk.rowinfo.resize(rowinfo.size());
+    const size_t ne_alloc = static_cast<size_t>(std::max<long>(total, 1));
     pk.pcol.resize(ne_alloc);
     pk.psrc.resize(ne_alloc);
     pk.kptr.resize(kptr.size());
     PILU_HIP(hipMemcpy(pk.rowinfo.get(), rowinfo.data(), sizeof(int) * rowinfo.size(), hipMemcpyHostToDevice));
-    PILU_HIP(hipMemcpy(pk.pcol.get(), pcol.get(), sizeof(int) * ne_alloc, hipMemcpyHostToDevice));
-    PILU_HIP(hipMemcpy(pk.psrc.get(), psrc.get(), sizeof(int) * ne_alloc, hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(pk.kptr.get(), kptr.data(), sizeof(int) * kptr.size(), hipMemcpyHostToDevice));
+    // slabs on the device (every slot is written, padding included)
+    {
+        hipStream_t st = e->stream;
+        DevBuf<int> kinv(std::max(n, 1)), kb0(std::max(n, 1)), grp_of(std::max(n, 1));
+        DevBuf<int> dbase(std::max(ngroups, 1)), dwidth(std::max(ngroups, 1));
+        PILU_HIP(hipMemcpy(dbase.get(), pk.h_base.data(), sizeof(int) * ngroups, hipMemcpyHostToDevice));
+        PILU_HIP(hipMemcpy(dwidth.get(), pk.h_width.data(), sizeof(int) * ngroups, hipMemcpyHostToDevice));
+        if (total == 0) {
+            const int zero = 0, minus1 = -1;
+            PILU_HIP(hipMemcpy(pk.pcol.get(), &zero, sizeof(int), hipMemcpyHostToDevice));
+            PILU_HIP(hipMemcpy(pk.psrc.get(), &minus1, sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (n > 0) {
+            k_pack_inv<<<grid_for(n, 256), 256, 0, st>>>(n, perm.get(), kinv.get());
+            if (nblk > 0) k_pack_block_start<<<nblk, 256, 0, st>>>(nblk, pk.kptr.get(), perm.get(), kb0.get());
+            if (ngroups > 0)
+                k_pack_group_of<<<grid_for(ngroups, 256), 256, 0, st>>>(ngroups, loff.get(), grp_of.get());
+            if (upper)
+                k_pack_fill<true><<<grid_for(n, 256), 256, 0, st>>>(
+                    n, perm.get(), grp_of.get(), loff.get(), dbase.get(), dwidth.get(), e->rp.get(), e->dg.get(),
+                    e->ci.get(), e->block_of.get(), kinv.get(), kb0.get(), pk.pcol.get(), pk.psrc.get());
+            else
+                k_pack_fill<false><<<grid_for(n, 256), 256, 0, st>>>(
+                    n, perm.get(), grp_of.get(), loff.get(), dbase.get(), dwidth.get(), e->rp.get(), e->dg.get(),
+                    e->ci.get(), e->block_of.get(), kinv.get(), kb0.get(), pk.pcol.get(), pk.psrc.get());
+            PILU_HIP(hipGetLastError());
+        }
+        PILU_HIP(hipStreamSynchronize(st));
+    }
     pk.version = e->part_version;
     pk.pass_g = 0;
     pk.factors = -1;
