@@ -105,14 +105,31 @@ __global__ __launch_bounds__(256) void k_aggregate(int n, const int* __restrict_
                                                    float* __restrict__ out, bool accumulate) {
     constexpr int R = 64 / D;
     const int lane = threadIdx.x & 63;
-    const long row = (static_cast<long>(blockIdx.x) * 4 + (threadIdx.x >> 6)) * R + lane / D;
     const int d = lane % D;
-    if (row >= n) return;
-    float acc = 0.f;
-    for (int e = rp[row]; e < rp[row + 1]; ++e) acc = fmaf(w[e], h[static_cast<long>(ci[e]) * D + d], acc);
-    float* o = out + row * D + d;
-    *o = accumulate ? *o + acc : acc;
+    const long step = static_cast<long>(gridDim.x) * 4 * R;
+    // rows strided over a capped grid; a row's edges are read four at a time
+    // and summed in edge order
+    for (long row = (static_cast<long>(blockIdx.x) * 4 + (threadIdx.x >> 6)) * R + lane / D; row < n; row += step) {
+        float acc = 0.f;
+        int e = rp[row];
+        const int e1 = rp[row + 1];
+        for (; e + 4 <= e1; e += 4) {
+            float we[4], he[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                we[q] = w[e + q];
+                he[q] = h[static_cast<long>(ci[e + q]) * D + d];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = fmaf(we[q], he[q], acc);
+        }
+        for (; e < e1; ++e) acc = fmaf(w[e], h[static_cast<long>(ci[e]) * D + d], acc);
+        float* o = out + row * D + d;
+        *o = accumulate ? *o + acc : acc;
+    }
 }
+
+constexpr int kAggBlocks = 4096;  // grid cap of k_aggregate (rows beyond it are strided)
 
 // Z = [H | M] W + b, Hout = relu(Z); one wave per 16 rows, two 16x16 output tiles.
 template <int D>
@@ -382,11 +399,11 @@ public:
         for (int l = 0; l < gnn::kLayers; ++l) {
             const float* hin = l == 0 ? feat_.get() : h_[l - 1].get();
             if (l == 0) {
-                gnn::k_aggregate<gnn::kFeat><<<ceil_div(n, 16), 256>>>(n, rp, ci, ew_.get(), hin, m_[0].get(), false);
+                gnn::k_aggregate<gnn::kFeat><<<std::min(ceil_div(n, 16), gnn::kAggBlocks), 256>>>(n, rp, ci, ew_.get(), hin, m_[0].get(), false);
                 gnn::k_dense_fwd<gnn::kFeat><<<wgs16, 256>>>(n, hin, m_[0].get(), wb_.get() + gnn::kOffW[0],
                                                              params_.get() + gnn::kOffB[0], z_[0].get(), h_[0].get());
             } else {
-                gnn::k_aggregate<gnn::kHid><<<ceil_div(n, 8), 256>>>(n, rp, ci, ew_.get(), hin, m_[l].get(), false);
+                gnn::k_aggregate<gnn::kHid><<<std::min(ceil_div(n, 8), gnn::kAggBlocks), 256>>>(n, rp, ci, ew_.get(), hin, m_[l].get(), false);
                 gnn::k_dense_fwd<gnn::kHid><<<wgs16, 256>>>(n, hin, m_[l].get(), wb_.get() + gnn::kOffW[l],
                                                             params_.get() + gnn::kOffB[l], z_[l].get(), h_[l].get());
             }
@@ -440,12 +457,12 @@ public:
             if (D == gnn::kFeat) {
                 gnn::k_dense_bwd_x<gnn::kFeat><<<wgs16, 256>>>(n, dh_.get(), z_[l].get(), wb_.get() + gnn::kOffW[l],
                                                                dh2_.get(), dm_.get());
-                gnn::k_aggregate<gnn::kFeat><<<ceil_div(n, 16), 256>>>(n, rpt, cit, ewt_.get(), dm_.get(), dh2_.get(),
+                gnn::k_aggregate<gnn::kFeat><<<std::min(ceil_div(n, 16), gnn::kAggBlocks), 256>>>(n, rpt, cit, ewt_.get(), dm_.get(), dh2_.get(),
                                                                        true);
             } else {
                 gnn::k_dense_bwd_x<gnn::kHid><<<wgs16, 256>>>(n, dh_.get(), z_[l].get(), wb_.get() + gnn::kOffW[l],
                                                               dh2_.get(), dm_.get());
-                gnn::k_aggregate<gnn::kHid><<<ceil_div(n, 8), 256>>>(n, rpt, cit, ewt_.get(), dm_.get(), dh2_.get(),
+                gnn::k_aggregate<gnn::kHid><<<std::min(ceil_div(n, 8), gnn::kAggBlocks), 256>>>(n, rpt, cit, ewt_.get(), dm_.get(), dh2_.get(),
                                                                      true);
             }
             std::swap(dh_, dh2_);
