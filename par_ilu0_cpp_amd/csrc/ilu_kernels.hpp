@@ -851,46 +851,46 @@ __global__ __launch_bounds__(kBlock) void k_factor_short(int n, const int* __res
 #pragma unroll
     for (int p = 0; p < P; ++p) ub[p] = uu[p] = kPending;
     bool done = !active;
+    int pn = 0;              // pivots applied so far (uniform in the group)
+    double dropped = 0.0;    // this lane's share of the fill updates outside the pattern
     while (true) {
         if (!done) {
-            bool ok = true;
+            // poll every pivot not applied yet, then apply the ready ones in
+            // ascending order: only the last pivot to arrive stays on the
+            // critical path
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                if (p < ndiag) {
+                if (p >= pn && p < ndiag) {
                     if (ub[p] == kPending) ub[p] = peek_bits(vout + dk[p]);
                     if (tgt[p] != -2 && uu[p] == kPending) uu[p] = peek_bits(vout + qe[p]);
-                    ok = ok && ub[p] != kPending && (tgt[p] == -2 || uu[p] != kPending);
                 }
             }
-            bool ready = group_all<G>(ok);
-            if (!ready && group_all<G>(wd.expired())) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                if (p == pn && p < ndiag) {
+                    if (!group_all<G>(ub[p] != kPending && (tgt[p] == -2 || uu[p] != kPending))) break;
+                    const double ukk = __longlong_as_double(ub[p]);
+                    const double lik = val[p] / ukk;
+                    if (ukk == 0.0 && t == 0) atomicOr(err, 1);
+                    const double u = __longlong_as_double(uu[p]);
+                    if (tgt[p] >= 0) val[tgt[p]] = fma(-u, lik, val[tgt[p]]);
+                    else if (tgt[p] == -1) dropped = fma(-u, lik, dropped);
+                    if (t == 0) val[p] = lik;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    ++pn;
+                }
+            }
+            if (pn < ndiag && group_all<G>(wd.expired())) {
                 if (t == 0) atomicOr(err, kErrTimeout);
                 for (int q = t; q < len; q += G) val[q] = __longlong_as_double(kQuietNan);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-#pragma unroll
-                for (int p = 0; p < P; ++p) ub[p] = uu[p] = kQuietNan;
-                ndiag = 0;
-                ready = true;
+                pn = ndiag;
             }
-            if (ready) {
-                double dropped = 0.0;
-#pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    if (p < ndiag) {
-                        const double ukk = __longlong_as_double(ub[p]);
-                        const double lik = val[p] / ukk;
-                        if (ukk == 0.0 && t == 0) atomicOr(err, 1);
-                        const double u = __longlong_as_double(uu[p]);
-                        if (tgt[p] >= 0) val[tgt[p]] = fma(-u, lik, val[tgt[p]]);
-                        else if (tgt[p] == -1) dropped = fma(-u, lik, dropped);
-                        if (t == 0) val[p] = lik;
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                    }
-                }
+            if (pn >= ndiag) {
                 if (omega) {
                     const double dsum = group_sum<G>(dropped);
                     if (t == 0) val[ndiag] = fma(static_cast<double>(omega[row]), dsum, val[ndiag]);
