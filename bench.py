@@ -109,7 +109,7 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "c3":
-        workload_c3()
+        workload_c3(sweep=args.sweep)
         return
     if args.workload == "c4":
         workload_c4()
@@ -298,11 +298,14 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def workload_c3(n=1_000_000, lof=2, m=40):
+def workload_c3(n=1_000_000, lof=2, m=40, sweep=""):
     """Random n = 1e6 matrix (12 off-diagonals per row within +-5000,
     diagonal 1.05 x the row's absolute sum), ILU(2), GMRES(40) to relative
-    residual 1e-8 from x0 = 0, FP64 and FP32 factors, default sweeps; one
-    solve each, with the FP32 solution's relative distance to the FP64 one."""
+    residual 1e-8 from x0 = 0, FP64 and FP32 factors; one solve each, with
+    the FP32 solution's relative distance to the FP64 one.  Sweeps: the
+    single-workgroup level walk with 64 lanes per row by default (this
+    pattern has ~11 rows per level), "auto" for the autotuner, or
+    MODE_L:LANES_L:MODE_U:LANES_U:BLOCK."""
     import numpy as np
     import torch
 
@@ -332,6 +335,16 @@ def workload_c3(n=1_000_000, lof=2, m=40):
             raise SystemExit("factorization failed")
         torch.cuda.synchronize()
         t_fac = time.perf_counter() - ts
+        spec = sweep or "4:64:4:64:0"
+        ts = time.perf_counter()
+        if spec == "auto":
+            ilu.Autotune(reps=1)
+        elif spec != "default":
+            ml, ll, mu, lu, blk = (int(v) for v in spec.split(":"))
+            ilu.SetSweepLanes(ll, lu)
+            ilu.SetSweepMode(ml, mu, blk)
+        torch.cuda.synchronize()
+        t_shape = time.perf_counter() - ts
         g = P.PreconditionedGmres()
         g.SetIluPreconditioner(ilu)
         prm = g.param()
@@ -346,7 +359,17 @@ def workload_c3(n=1_000_000, lof=2, m=40):
         torch.cuda.synchronize()
         t_sol = time.perf_counter() - ts
         sols[bits] = x.cpu().numpy()
+        apply_s = []
+        if spec != "auto":  # time two preconditioner applies on their own
+            xb = torch.empty_like(x)
+            for _ in range(3):
+                torch.cuda.synchronize()
+                ta = time.perf_counter()
+                ilu.SubstituteDevice(d["b"].data_ptr(), xb.data_ptr())
+                torch.cuda.synchronize()
+                apply_s.append(time.perf_counter() - ta)
         out[f"fp{bits}"] = {"nnz_f": nnz_f, "levels": list(ilu.Levels()), "symbolic_s": t_sym, "factor_s": t_fac,
+                            "sweep_setup_s": t_shape, "apply_s": min(apply_s[1:]) if apply_s else None,
                             "solve_s": t_sol, "its": its, "ok": bool(ok),
                             "true_rel_res": g.LastTrueResidual() / float(np.linalg.norm(b)),
                             "sweep": ilu.SweepConfig()}

@@ -70,6 +70,12 @@ struct lljbash::IluSolver::Ext {
     pilu::DevBuf<int> rp, ci, dg;
     pilu::DevBuf<int> perm_l, perm_u, level;
     int nlev_l = 0, nlev_u = 0;
+    pilu::DevBuf<int> lvoff_l, lvoff_u;  // first position of every level in perm_l / perm_u (+ n)
+    // per-slot row descriptors of the single-workgroup sweep (mode 4), built
+    // on first use for (pattern_version, slots per level)
+    pilu::DevBuf<int4> lvdesc_l, lvdesc_u;
+    int lvkey_l[2] = {-1, 0}, lvkey_u[2] = {-1, 0};
+    int pattern_version = 0;
 
     pilu::DevBuf<double> vin;    // values to factorize (scratch)
     pilu::DevBuf<double> fac;    // factors, FP64
@@ -88,7 +94,9 @@ struct lljbash::IluSolver::Ext {
     int lanes_l = 0, lanes_u = 0;  // 0 = heuristic
     // Sweep algorithm per direction: 0 = level-scheduled rows (k_sweep),
     // 1 = block-local over contiguous blocks of blk_rows rows (k_sweep_blk),
-    // 2 = packed block-local over the current partition (k_sweep_pp).
+    // 2 = packed block-local over the current partition (k_sweep_pp, level
+    // barriers), 3 = the same with LDS value-flags, 4 = one workgroup walking
+    // the level schedule (k_sweep_lv; no partition).
     int mode_l = 0, mode_u = 0;
     pilu::SweepPack pack_l, pack_u;
     // Block partition of the rows: contiguous blocks of blk_rows rows, or

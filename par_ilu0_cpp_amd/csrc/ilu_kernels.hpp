@@ -367,6 +367,175 @@ __global__ __launch_bounds__(512) void k_sweep_blk(int n, int blk, int nblk, con
 }
 
 // ---------------------------------------------------------------------
+// Single-workgroup level walk.  One workgroup of kLvThreads threads solves
+// every row, level by level (perm sorted by level: level l is
+// perm[lev_off[l] .. lev_off[l+1])), with a workgroup barrier between
+// levels.  All dependencies of a row sit in earlier levels and were written
+// by this workgroup before a barrier, so there are no value-flag waits and
+// no hand-offs between workgroups: a level costs one barrier and one read of
+// its dependencies' x, which hit lines this workgroup wrote.  It suits deep
+// dependency graphs with few rows per level (random sparsity with fill),
+// where the hand-off latency of the multi-workgroup sweeps dominates.
+//
+// Slot j of level l (its j-th row, j < kLvThreads / G) is described by
+// desc[l * slots + j] = {row or -1, first entry, end, rows in level l}.  The
+// descriptor is read four levels ahead and the slot's row data (the lane's first kLvH
+// entries, columns and values; b; the diagonal) two levels ahead, in two
+// register sets used alternately, from clamped addresses so that the issue
+// sequence has no branches: loads complete in order, and a level then
+// waits only for its x reads (the loop is unrolled four times, so the
+// register sets are carried around the loop, which waits for the
+// outstanding reads, once per four levels).  Rows past the first `slots` of
+// a level are read directly.  Lane t of a G-lane group sums entries t, t+G, ... in order
+// from b (lane 0) and the group total is k_sweep's butterfly, so results
+// are bit-identical to k_sweep with the same G.  Like k_sweep, each row
+// stores kPending into reset[row] after reading b[row].
+// ---------------------------------------------------------------------
+constexpr int kLvThreads = 1024;
+constexpr int kLvH = 4;
+
+template <typename T>
+struct LvRow {
+    int row, e, end, w;  // e: this lane's first entry; w: rows in the level
+    int c[kLvH];
+    T v[kLvH];
+    double bv;
+    T dv;
+};
+
+template <int G, typename T, bool UPPER>
+__device__ __forceinline__ void lv_fetch(const int4& d, int t, long last, const int* __restrict__ ci,
+                                         const T* __restrict__ val, const double* b, LvRow<T>& r) {
+    r.row = d.x;
+    r.e = d.y + t;
+    r.end = d.z;
+    r.w = d.w;
+    // positions past the row's end read its last entry again (a line already
+    // requested), not the next rows' entries: the single workgroup's
+    // requests in flight are the limit, so none are spent on unused data
+    long hi = d.z > d.y ? d.z - 1 : d.y;
+    hi = hi < last ? hi : last;
+#pragma unroll
+    for (int q = 0; q < kLvH; ++q) {
+        long k = static_cast<long>(r.e) + q * G;
+        k = k < hi ? k : hi;
+        r.c[q] = ci[k];
+        r.v[q] = val[k];
+    }
+    r.bv = b[d.x >= 0 ? d.x : 0];
+    if (UPPER) r.dv = val[d.y > 0 ? d.y - 1 : 0];
+    else r.dv = T(1);
+}
+
+// One level: solves the slot's row from `r` (fetched two levels earlier),
+// refills `r` with level l + 2 from `d` and `d` with the descriptor of
+// level l + 4, then the level's further rows, then the barrier.
+template <int G, typename T, bool UPPER>
+__device__ __forceinline__ void lv_level(int l, int nlev, LvRow<T>& r, int4& d, const int* __restrict__ lev_off,
+                                         const int4* __restrict__ desc, const int* __restrict__ perm,
+                                         const int* __restrict__ rp, const int* __restrict__ ci,
+                                         const int* __restrict__ dg, const T* __restrict__ val, long last,
+                                         const double* b, double* out, double* reset) {
+    constexpr int kSlots = kLvThreads / G;
+    const int j = static_cast<int>(threadIdx.x) / G;
+    const int t = threadIdx.x & (G - 1);
+    const LvRow<T> cur = r;
+    double x[kLvH];
+#pragma unroll
+    for (int q = 0; q < kLvH; ++q) x[q] = out[cur.c[q]];
+    lv_fetch<G, T, UPPER>(d, t, last, ci, val, b, r);
+    {
+        const int l4 = l + 4 < nlev ? l + 4 : nlev - 1;
+        d = desc[static_cast<long>(l4) * kSlots + j];
+    }
+    double acc = t == 0 ? cur.bv : 0.0;
+#pragma unroll
+    for (int q = 0; q < kLvH; ++q)
+        if (cur.e + q * G < cur.end) acc = fma(-x[q], static_cast<double>(cur.v[q]), acc);
+    for (int e = cur.e + kLvH * G; e < cur.end; e += G) acc = fma(-out[ci[e]], static_cast<double>(val[e]), acc);
+    const double s = group_sum<G>(acc);
+    if (t == 0 && cur.row >= 0) {
+        out[cur.row] = UPPER ? s / static_cast<double>(cur.dv) : s;
+        if (reset) reinterpret_cast<uint64_t*>(reset)[cur.row] = kPending;
+    }
+    if (cur.w > kSlots) {
+        const int p0 = lev_off[l], p1 = p0 + cur.w;
+        for (int p = p0 + kSlots + j; p < p1; p += kSlots) {
+            const int row = perm[p];
+            const int dd = dg[row];
+            const int e0 = UPPER ? dd + 1 : rp[row], end = UPPER ? rp[row + 1] : dd;
+            double a = t == 0 ? b[row] : 0.0;
+            for (int e = e0 + t; e < end; e += G) a = fma(-out[ci[e]], static_cast<double>(val[e]), a);
+            const double sm = group_sum<G>(a);
+            if (t == 0) {
+                out[row] = UPPER ? sm / static_cast<double>(val[dd]) : sm;
+                if (reset) reinterpret_cast<uint64_t*>(reset)[row] = kPending;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int G, typename T, bool UPPER>
+__global__ __launch_bounds__(kLvThreads) void k_sweep_lv(int nlev, const int* __restrict__ lev_off,
+                                                         const int4* __restrict__ desc, const int* __restrict__ perm,
+                                                         const int* __restrict__ rp, const int* __restrict__ ci,
+                                                         const int* __restrict__ dg, const T* __restrict__ val,
+                                                         long nnz, const double* b, double* out, double* reset,
+                                                         const int* gate) {
+    if (gate && *gate == 0) return;
+    if (nlev <= 0 || nnz <= 0) return;
+    constexpr int kSlots = kLvThreads / G;
+    const int j = static_cast<int>(threadIdx.x) / G;
+    const int t = threadIdx.x & (G - 1);
+    const long last = nnz - 1;
+    auto dsc = [&](int l) { return desc[static_cast<long>(l < nlev ? l : nlev - 1) * kSlots + j]; };
+    LvRow<T> r0, r1;
+    lv_fetch<G, T, UPPER>(dsc(0), t, last, ci, val, b, r0);
+    lv_fetch<G, T, UPPER>(dsc(1), t, last, ci, val, b, r1);
+    int4 d0 = dsc(2), d1 = dsc(3);
+    for (int l = 0; l < nlev; l += 4) {
+        lv_level<G, T, UPPER>(l, nlev, r0, d0, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+        if (l + 1 >= nlev) break;
+        lv_level<G, T, UPPER>(l + 1, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+        if (l + 2 >= nlev) break;
+        lv_level<G, T, UPPER>(l + 2, nlev, r0, d0, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+        if (l + 3 >= nlev) break;
+        lv_level<G, T, UPPER>(l + 3, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+    }
+}
+
+// desc[l * slots + j] of k_sweep_lv: the j-th row of level l, its first
+// entry and end in the sweep's direction (row -1, no entries, past the
+// level's last row), and the level's row count.
+template <bool UPPER>
+__global__ void k_lv_desc(int nlev, int slots, const int* __restrict__ lev_off, const int* __restrict__ perm,
+                          const int* __restrict__ rp, const int* __restrict__ dg, int4* __restrict__ desc) {
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<long>(nlev) * slots) return;
+    const int l = static_cast<int>(i / slots), j = static_cast<int>(i % slots);
+    const int p = lev_off[l] + j;
+    int4 d = make_int4(-1, 0, 0, lev_off[l + 1] - lev_off[l]);
+    if (p < lev_off[l + 1]) {
+        const int row = perm[p];
+        d.x = row;
+        d.y = UPPER ? dg[row] + 1 : rp[row];
+        d.z = UPPER ? rp[row + 1] : dg[row];
+    }
+    desc[i] = d;
+}
+
+// off[l] = first position of level l in a level-sorted order (every level
+// from 0 to nlev - 1 holds at least one row); off[nlev] = n.
+__global__ void k_level_starts(int n, int nlev, const int* __restrict__ lev_sorted, int* __restrict__ off) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (k == 0) off[0] = 0;
+    else if (lev_sorted[k] != lev_sorted[k - 1]) off[lev_sorted[k]] = k;
+    if (k == n - 1) off[nlev] = n;
+}
+
+// ---------------------------------------------------------------------
 // Packed block-local sweep.  The rows are split into blocks (contiguous row
 // ranges, or any partition whose block dependency graph is acyclic, such
 // as 3D tiles of a grid); each workgroup takes one block in a topological

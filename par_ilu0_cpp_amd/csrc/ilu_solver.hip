@@ -137,7 +137,7 @@ void launch_levels(Ext* e, const int* block_of, int* level, int* maxlev, uint64_
 }
 
 // Levels of one dependency graph and the rows sorted by (level, row).
-int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
+int build_schedule(Ext* e, bool upper, DevBuf<int>& perm, DevBuf<int>& lev_off) {
     const int n = e->n;
     hipStream_t s = e->stream;
     int* level = e->level.get();
@@ -165,6 +165,12 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm) {
     e->sort_tmp.reserve(bytes);
     PILU_HIP(rocprim::radix_sort_pairs(e->sort_tmp.get(), bytes, level, keys_out.get(), rows.get(),
                                        perm.get(), static_cast<unsigned>(n), 0u, end_bit, s));
+    lev_off.reserve(static_cast<size_t>(nlev) + 1);
+    // every level 0..nlev-1 holds a row (a row's level is one more than its
+    // deepest dependency's); the fill only keeps the array defined if not
+    k_fill_int<<<grid_for(static_cast<long>(nlev) + 1, 256), 256, 0, s>>>(static_cast<long>(nlev) + 1, lev_off.get(), n);
+    k_level_starts<<<grid_for(n, 256), 256, 0, s>>>(n, nlev, keys_out.get(), lev_off.get());
+    PILU_HIP(hipGetLastError());
     PILU_HIP(hipStreamSynchronize(s));
     return nlev;
 }
@@ -493,8 +499,9 @@ void analyse(Ext* e, const int* host_rp) {
     e->max_lo = max_lo;
     e->max_up = max_up;
 
-    e->nlev_l = build_schedule(e, false, e->perm_l);
-    e->nlev_u = build_schedule(e, true, e->perm_u);
+    e->nlev_l = build_schedule(e, false, e->perm_l, e->lvoff_l);
+    e->nlev_u = build_schedule(e, true, e->perm_u, e->lvoff_u);
+    ++e->pattern_version;
 
     e->vin.reserve(e->nnz);
     e->fac.reserve(e->nnz);
@@ -838,10 +845,47 @@ void sweep_pp_dispatch(int lanes, Ext* e, const double* b, double* out, double* 
     }
 }
 
+template <int G, typename T, bool UPPER>
+void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* reset, const int* gate) {
+    constexpr int slots = kLvThreads / G;
+    const int nlev = UPPER ? e->nlev_u : e->nlev_l;
+    if (e->n <= 0 || nlev <= 0) return;
+    DevBuf<int4>& desc = UPPER ? e->lvdesc_u : e->lvdesc_l;
+    int* key = UPPER ? e->lvkey_u : e->lvkey_l;
+    const DevBuf<int>& off = UPPER ? e->lvoff_u : e->lvoff_l;
+    const DevBuf<int>& perm = UPPER ? e->perm_u : e->perm_l;
+    const long nd = static_cast<long>(nlev) * slots;
+    if (off.size() < static_cast<size_t>(nlev) + 1 || perm.size() < static_cast<size_t>(e->n))
+        throw std::runtime_error("level-walk sweep: level schedule missing");
+    if (key[0] != e->pattern_version || key[1] != slots) {
+        desc.reserve(static_cast<size_t>(nd));
+        k_lv_desc<UPPER><<<grid_for(nd, 256), 256, 0, e->stream>>>(nlev, slots, off.get(), perm.get(), e->rp.get(),
+                                                                  e->dg.get(), desc.get());
+        PILU_HIP(hipGetLastError());
+        key[0] = e->pattern_version;
+        key[1] = slots;
+    }
+    k_sweep_lv<G, T, UPPER><<<1, kLvThreads, 0, e->stream>>>(nlev, off.get(), desc.get(), perm.get(), e->rp.get(),
+                                                            e->ci.get(), e->dg.get(), val, e->nnz, b, out, reset,
+                                                            gate);
+}
+
 template <typename T, bool UPPER>
 void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* out, double* reset,
                     const int* gate, int* ctr) {
     const int mode = UPPER ? e->mode_u : e->mode_l;
+    if (mode == 4) {
+        switch (lanes) {
+            case 1: sweep_lv_one<1, T, UPPER>(e, val, b, out, reset, gate); break;
+            case 2: sweep_lv_one<2, T, UPPER>(e, val, b, out, reset, gate); break;
+            case 4: sweep_lv_one<4, T, UPPER>(e, val, b, out, reset, gate); break;
+            case 8: sweep_lv_one<8, T, UPPER>(e, val, b, out, reset, gate); break;
+            case 16: sweep_lv_one<16, T, UPPER>(e, val, b, out, reset, gate); break;
+            case 32: sweep_lv_one<32, T, UPPER>(e, val, b, out, reset, gate); break;
+            default: sweep_lv_one<64, T, UPPER>(e, val, b, out, reset, gate); break;
+        }
+        return;
+    }
     if (mode >= 2 && e->nblk > 0) {
         ensure_packs(e);
         sweep_pp_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
@@ -1130,6 +1174,10 @@ void time_sweeps(Ext* e, int lanes, const double* b, double* x, int reps, hipEve
 
 namespace lljbash {
 
+// Average rows per level up to which the autotuner tries the
+// single-workgroup level walk (mode 4).
+constexpr int kLvTryRows = 256;
+
 int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     Ext* e = pilu::need_setup(ext_);
     if (!e->factored) throw std::runtime_error("IluSolver: Factorize before AutotuneSweeps");
@@ -1176,9 +1224,24 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
         if (tl < lev_best[0]) { lev_best[0] = tl; lev_lanes[0] = g; }
         if (tu < lev_best[1]) { lev_best[1] = tu; lev_lanes[1] = g; }
     }
+    // one workgroup walking the levels, where a level averages few rows
+    // (a level wider than the workgroup's row slots is read without
+    // prefetch, so wide levels are not tried)
+    int lev_mode[2] = {0, 0};
+    const int deep = std::max(std::max(e->nlev_l, e->nlev_u), 1);
+    if (e->n / deep <= kLvTryRows) {
+        e->mode_l = e->mode_u = 4;
+        for (int g : {4, 8, 16, 32, 64}) {
+            double tl, tu;
+            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+            record(4, Part{0, 0, 0, 0}, g, tl, tu);
+            if (tl < lev_best[0]) { lev_best[0] = tl; lev_lanes[0] = g; lev_mode[0] = 4; }
+            if (tu < lev_best[1]) { lev_best[1] = tu; lev_lanes[1] = g; lev_mode[1] = 4; }
+        }
+    }
     // block-local sweeps, per partition
     double best_total = lev_best[0] + lev_best[1];
-    int pick_mode[2] = {0, 0}, pick_lanes[2] = {lev_lanes[0], lev_lanes[1]};
+    int pick_mode[2] = {lev_mode[0], lev_mode[1]}, pick_lanes[2] = {lev_lanes[0], lev_lanes[1]};
     int pick = -1;
     for (size_t ip = 0; ip < parts.size(); ++ip) {
         const Part& pt = parts[ip];
@@ -1212,7 +1275,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
             best_total = tot;
             pick = static_cast<int>(ip);
             for (int d = 0; d < 2; ++d) {
-                pick_mode[d] = bb[d] < lev_best[d] ? bm[d] : 0;
+                pick_mode[d] = bb[d] < lev_best[d] ? bm[d] : lev_mode[d];
                 pick_lanes[d] = bb[d] < lev_best[d] ? bl[d] : lev_lanes[d];
             }
         }
@@ -1228,18 +1291,20 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     e->mode_u = pick_mode[1];
     e->lanes_l = pick_lanes[0];
     e->lanes_u = pick_lanes[1];
-    if (pick_mode[0] >= 2 || pick_mode[1] >= 2) pilu::ensure_packs(e);
+    if ((pick_mode[0] >= 2 && pick_mode[0] <= 3) || (pick_mode[1] >= 2 && pick_mode[1] <= 3)) pilu::ensure_packs(e);
     pilu::show_algorithm(this);
     return rows;
 }
 
 void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
     Ext* e = pilu::need_setup(ext_);
-    if (mode_l < 0 || mode_l > 3 || mode_u < 0 || mode_u > 3) throw std::runtime_error("sweep mode is 0, 1, 2 or 3");
-    if (mode_l > 0 || mode_u > 0) pilu::use_blocks(e, blk_rows);
+    if (mode_l < 0 || mode_l > 4 || mode_u < 0 || mode_u > 4)
+        throw std::runtime_error("sweep mode is 0, 1, 2, 3 or 4");
+    auto blocked = [](int m) { return m >= 1 && m <= 3; };
+    if (blocked(mode_l) || blocked(mode_u)) pilu::use_blocks(e, blk_rows);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
-    if (mode_l >= 2 || mode_u >= 2) pilu::ensure_packs(e);
+    if ((mode_l >= 2 && mode_l <= 3) || (mode_u >= 2 && mode_u <= 3)) pilu::ensure_packs(e);
     pilu::show_algorithm(this);
 }
 

@@ -90,7 +90,7 @@ def test_sweeps_all_lane_shapes(oracle, bits):
 
 
 @pytest.mark.parametrize("bits", [64, 32])
-@pytest.mark.parametrize("mode,blk", [(0, 0), (1, 512), (2, 512), (2, 4096), (3, 512), (3, 4096)])
+@pytest.mark.parametrize("mode,blk", [(0, 0), (1, 512), (2, 512), (2, 4096), (3, 512), (3, 4096), (4, 0)])
 def test_sweeps_bit_exact_in_lane_order(oracle, bits, mode, blk):
     """Every sweep kernel, at every lanes-per-row shape, reproduces the
     oracle's lane-order sweep bit for bit (FP32 factors: the oracle gets the
@@ -105,7 +105,7 @@ def test_sweeps_bit_exact_in_lane_order(oracle, bits, mode, blk):
     if mode:
         ilu.SetSweepMode(mode, mode, blk)
     rng = np.random.default_rng(mode * 10 + bits)
-    for g in (LANES if mode <= 1 else (1, 2, 4, 8, 16)):
+    for g in (LANES if mode in (0, 1, 4) else (1, 2, 4, 8, 16)):
         ilu.SetSweepLanes(g, g)
         for _ in range(2):
             b = rng.uniform(-1, 1, a.n)
@@ -378,3 +378,87 @@ def test_tiles_need_a_grid_and_follow_the_dependencies():
             else:
                 U[i, j] = va[p]
     assert np.allclose(L @ (U @ x), b, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("case", ["poisson64", "rand_deep", "rand_long"])
+def test_level_walk_sweep_bit_exact(oracle, bits, case):
+    """The single-workgroup level walk (mode 4) against the oracle's
+    lane-order sweep, bit for bit, at every lanes-per-row shape: levels
+    wider than the workgroup's row slots (2D Poisson, up to 64 rows per
+    level), a deep random ILU(2) pattern (~13 rows per level, rows of ~100
+    entries: the 4-entry-per-lane head and the tail), and rows of ~150
+    entries per direction.  Mixed with the other modes per direction and
+    applied repeatedly (re-armed buffers)."""
+    if case == "poisson64":
+        a = gen.poisson2d(64)
+        rp, ci, va = a.row_ptr, a.col_idx, a.value
+    elif case == "rand_deep":
+        rp0, ci0 = random_pattern(6000, 12, 300, seed=5)
+        va0 = dominant_values(rp0, ci0, seed=5)
+        o = oracle.iluk_symbolic(rp0, ci0, va0, 2)
+        rp, ci, va = o[0], o[1], o[3]
+    else:
+        rp, ci = random_pattern(1200, 150, 400, seed=7)
+        va = dominant_values(rp, ci, seed=7)
+    n = rp.size - 1
+    ilu = factored(rp, ci, va, bits)
+    ofac, dg, _ = oracle.ilu_factorize(rp, ci, va)
+    if bits == 32:
+        ofac = ofac.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(bits + len(case))
+    for g in LANES:
+        ilu.SetSweepLanes(g, g)
+        ilu.SetSweepMode(4, 4, 0)
+        assert ilu.SweepConfig()["mode_l"] == 4 and ilu.SweepConfig()["mode_u"] == 4
+        for _ in range(2):
+            b = rng.uniform(-1, 1, n)
+            assert np.array_equal(ilu.Substitute(b), oracle.substitute_lanes(rp, ci, dg, ofac, b, g)), g
+    b = rng.uniform(-1, 1, n)
+    for g in (4, 16):
+        ilu.SetSweepLanes(g, g)
+        ilu.SetSweepMode(0, 0, 0)
+        x_level = ilu.Substitute(b)
+        for ml, mu, blk in ((4, 0, 0), (0, 4, 0), (4, 2, 512), (2, 4, 512), (4, 4, 0)):
+            ilu.SetSweepMode(ml, mu, blk)
+            for _ in range(2):
+                assert np.array_equal(ilu.Substitute(b), x_level), (g, ml, mu)
+
+
+def test_level_walk_follows_new_pattern_and_values(oracle):
+    """Mode 4's per-level descriptors are rebuilt for a new pattern and the
+    sweep reads refactorized values."""
+    ilu = P.IluSolver(0)
+    for s, scale in ((30, 1.0), (41, 1.3)):
+        a = gen.poisson2d(s)
+        va = a.value * np.linspace(1.0, scale, a.nnz)
+        ilu.SetMatrix(a.row_ptr, a.col_idx, va)
+        assert ilu.SetupMatrix() == 0 and ilu.Factorize() == 0
+        ilu.SetSweepLanes(8, 8)
+        ilu.SetSweepMode(4, 4, 0)
+        ofac, dg, _ = oracle.ilu_factorize(a.row_ptr, a.col_idx, va)
+        b = np.random.default_rng(s).uniform(-1, 1, a.n)
+        assert np.array_equal(ilu.Substitute(b), oracle.substitute_lanes(a.row_ptr, a.col_idx, dg, ofac, b, 8))
+        va2 = va * 1.1
+        P.fill_csr(ilu.GetMatrix(), a.row_ptr, a.col_idx, va2)
+        assert ilu.Factorize() == 0
+        ofac2, _, _ = oracle.ilu_factorize(a.row_ptr, a.col_idx, va2)
+        assert np.array_equal(ilu.Substitute(b), oracle.substitute_lanes(a.row_ptr, a.col_idx, dg, ofac2, b, 8))
+
+
+def test_autotune_tries_level_walk_on_deep_patterns(oracle):
+    """On a pattern with few rows per level the autotuner times mode 4 and
+    whatever it picks keeps the results within 1e-12 of the oracle."""
+    rp0, ci0 = random_pattern(6000, 12, 300, seed=9)
+    va0 = dominant_values(rp0, ci0, seed=9)
+    o = oracle.iluk_symbolic(rp0, ci0, va0, 2)
+    rp, ci, va = o[0], o[1], o[3]
+    ilu = factored(rp, ci, va)
+    nl, nu = ilu.Levels()
+    assert (rp.size - 1) / max(nl, nu) <= 256
+    cfg, table = ilu.Autotune(reps=1)
+    assert any(int(r[0]) == 4 for r in table)
+    ofac, dg, _ = oracle.ilu_factorize(rp, ci, va)
+    b = np.random.default_rng(1).uniform(-1, 1, rp.size - 1)
+    xo = oracle.substitute(rp, ci, dg, ofac, b)
+    assert np.max(np.abs(ilu.Substitute(b) - xo)) <= 1e-12 * np.max(np.abs(xo))

@@ -10,6 +10,7 @@ import pytest
 
 import par_ilu0_cpp_amd as P
 from par_ilu0_cpp_amd import generators as gen
+from tests.conftest import dominant_values, random_pattern
 
 pytestmark = pytest.mark.gpu
 
@@ -178,3 +179,30 @@ def test_large_convdiff_reaches_tolerance(side):
     assert ok and its > 0
     res = np.linalg.norm(b - a.matvec(x.cpu().numpy())) / bn
     assert res <= 1.01e-8
+
+
+def test_gmres_with_level_walk_sweeps_matches_level_sweeps(oracle):
+    """GMRES with the single-workgroup level walk (mode 4) takes the same
+    steps to the same bits as with the level-scheduled sweep (same lanes),
+    and matches the oracle's step count within 1."""
+    rp0, ci0 = random_pattern(8000, 12, 400, seed=3)
+    va0 = dominant_values(rp0, ci0, seed=3)
+    am = P.HostCsr(rp0, ci0, va0)
+    o = oracle.iluk_symbolic(rp0, ci0, va0, 2)
+    b = np.random.default_rng(3).uniform(-1, 1, rp0.size - 1)
+    tol = 1e-10 * float(np.linalg.norm(b))
+    runs = []
+    for mode in (0, 4):
+        ilu = solver_for(o[0], o[1], o[3])
+        ilu.SetSweepLanes(16, 16)
+        ilu.SetSweepMode(mode, mode, 0)
+        g = gmres(ilu, tol, 40, 400)
+        x = np.zeros(rp0.size - 1)
+        ok, its = g.Solve(am, b, x)
+        assert ok
+        runs.append((its, x))
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
+    fac, dg, _ = oracle.ilu_factorize(o[0], o[1], o[3])
+    ok_o, its_o, _, _ = oracle.gmres((rp0, ci0, va0), b, tol, 40, 400, factors=(o[0], o[1], dg, fac))
+    assert ok_o and abs(runs[1][0] - its_o) <= 1
