@@ -75,6 +75,8 @@ struct lljbash::IluSolver::Ext {
     // on first use for (pattern_version, slots per level)
     pilu::DevBuf<int4> lvdesc_l, lvdesc_u;
     int lvkey_l[2] = {-1, 0}, lvkey_u[2] = {-1, 0};
+    pilu::DevBuf<unsigned long long> lvprog;  // [0]: progress word of the mode-4 sweep; [1]: sink
+    unsigned lv_epoch = 0;
     int pattern_version = 0;
 
     pilu::DevBuf<double> vin;    // values to factorize (scratch)

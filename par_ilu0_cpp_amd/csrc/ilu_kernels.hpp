@@ -476,15 +476,86 @@ __device__ __forceinline__ void lv_level(int l, int nlev, LvRow<T>& r, int4& d, 
     __syncthreads();
 }
 
+// L2 warming for k_sweep_lv.  The solving workgroup's reads of the factors
+// are limited by the requests one compute unit keeps in flight, each
+// waiting a full memory latency.  Helper workgroups on the solver's XCD
+// (workgroups are dealt to the 8 XCDs round-robin, so workgroups 8, 16, ...
+// share workgroup 0's L2) read the lines of the levels kLvAheadMin to
+// kLvAheadMax ahead of the solver's progress (published every four
+// levels), helper k taking every helpers-th level; the solver then finds
+// them in L2.  They read only, and write `sink` only if a checksum takes
+// an impossible value (keeping the reads), so results do not depend on
+// them; each leaves when the solver is done, when the next level is past
+// the last, or when its watchdog expires.  `epoch` tells this launch's
+// progress word from an earlier launch's.
+constexpr int kLvHelpers = 7;
+constexpr int kLvAheadMin = 4;
+constexpr int kLvAheadMax = 64;
+
+template <int G, bool UPPER, typename T>
+__device__ void lv_helper(int k, int nlev, const int* __restrict__ lev_off, const int4* __restrict__ desc,
+                          const int* __restrict__ perm, const int* __restrict__ rp, const int* __restrict__ dg,
+                          const int* __restrict__ ci, const T* __restrict__ val, const double* b,
+                          const unsigned long long* prog, unsigned epoch, int* sink, uint64_t wd_limit) {
+    constexpr int kSlots = kLvThreads / G;
+    __shared__ int s_lvl;
+    const Watchdog wd(wd_limit);
+    const int tid = threadIdx.x;
+    int next = k;
+    int acc = 0;
+    double accd = 0.0;
+    while (next < nlev) {
+        if (tid == 0) {
+            const unsigned long long p = __hip_atomic_load(prog, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_lvl = static_cast<unsigned>(p >> 32) == epoch ? static_cast<int>(p & 0xffffffffu) : 0;
+            if (wd.expired()) s_lvl = nlev;
+        }
+        __syncthreads();
+        const int lvl = s_lvl;
+        __syncthreads();
+        if (lvl >= nlev) break;
+        while (next < lvl + kLvAheadMin) next += kLvHelpers;
+        if (next >= nlev) break;
+        if (next >= lvl + kLvAheadMax) {
+            spin_pause();
+            continue;
+        }
+        // the level's descriptors, and per row one read per 64 bytes of its
+        // columns and values (16 lanes per row, rows 64 at a time)
+        if (tid < kSlots) acc ^= desc[static_cast<long>(next) * kSlots + tid].x;
+        const int p0 = lev_off[next], p1 = lev_off[next + 1];
+        for (int p = p0 + tid / 16; p < p1; p += kLvThreads / 16) {
+            const int row = perm[p];
+            const int d = dg[row];
+            const int e0 = UPPER ? d : rp[row], e1 = UPPER ? rp[row + 1] : d;
+            if ((tid & 15) == 0) accd += b[row];
+            for (int e = e0 + (tid & 15) * 16; e < e1; e += 256) {
+                acc ^= ci[e];
+                accd += static_cast<double>(val[e]);
+                if (sizeof(T) == 8 && e + 8 < e1) accd += static_cast<double>(val[e + 8]);
+            }
+        }
+        next += kLvHelpers;
+    }
+    if (acc == 0x7fffffff && accd == -1.0) sink[0] = 1;
+}
+
 template <int G, typename T, bool UPPER>
 __global__ __launch_bounds__(kLvThreads) void k_sweep_lv(int nlev, const int* __restrict__ lev_off,
                                                          const int4* __restrict__ desc, const int* __restrict__ perm,
                                                          const int* __restrict__ rp, const int* __restrict__ ci,
                                                          const int* __restrict__ dg, const T* __restrict__ val,
                                                          long nnz, const double* b, double* out, double* reset,
-                                                         const int* gate) {
+                                                         const int* gate, unsigned long long* prog, unsigned epoch,
+                                                         int* sink, uint64_t wd_limit) {
     if (gate && *gate == 0) return;
     if (nlev <= 0 || nnz <= 0) return;
+    if (blockIdx.x != 0) {
+        if (blockIdx.x % 8 == 0 && static_cast<int>(blockIdx.x / 8) <= kLvHelpers)
+            lv_helper<G, UPPER, T>(static_cast<int>(blockIdx.x / 8) - 1, nlev, lev_off, desc, perm, rp, dg, ci, val,
+                                   b, prog, epoch, sink, wd_limit);
+        return;
+    }
     constexpr int kSlots = kLvThreads / G;
     const int j = static_cast<int>(threadIdx.x) / G;
     const int t = threadIdx.x & (G - 1);
@@ -494,7 +565,9 @@ __global__ __launch_bounds__(kLvThreads) void k_sweep_lv(int nlev, const int* __
     lv_fetch<G, T, UPPER>(dsc(0), t, last, ci, val, b, r0);
     lv_fetch<G, T, UPPER>(dsc(1), t, last, ci, val, b, r1);
     int4 d0 = dsc(2), d1 = dsc(3);
+    const unsigned long long ep = static_cast<unsigned long long>(epoch) << 32;
     for (int l = 0; l < nlev; l += 4) {
+        if (threadIdx.x == 0) __hip_atomic_store(prog, ep | static_cast<unsigned>(l), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         lv_level<G, T, UPPER>(l, nlev, r0, d0, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
         if (l + 1 >= nlev) break;
         lv_level<G, T, UPPER>(l + 1, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
@@ -503,6 +576,7 @@ __global__ __launch_bounds__(kLvThreads) void k_sweep_lv(int nlev, const int* __
         if (l + 3 >= nlev) break;
         lv_level<G, T, UPPER>(l + 3, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
     }
+    if (threadIdx.x == 0) __hip_atomic_store(prog, ep | static_cast<unsigned>(nlev), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // desc[l * slots + j] of k_sweep_lv: the j-th row of level l, its first

@@ -865,9 +865,17 @@ void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* re
         key[0] = e->pattern_version;
         key[1] = slots;
     }
-    k_sweep_lv<G, T, UPPER><<<1, kLvThreads, 0, e->stream>>>(nlev, off.get(), desc.get(), perm.get(), e->rp.get(),
-                                                            e->ci.get(), e->dg.get(), val, e->nnz, b, out, reset,
-                                                            gate);
+    if (e->lvprog.size() < 2) {
+        e->lvprog.reserve(2);
+        PILU_HIP(hipMemset(e->lvprog.get(), 0, 2 * sizeof(unsigned long long)));
+    }
+    // workgroup 0 solves; workgroups 8, 16, ..., 8 * kLvHelpers (on its XCD)
+    // warm its L2; the others return at once
+    const int grid = 1 + 8 * kLvHelpers;
+    const unsigned epoch = ++e->lv_epoch;
+    k_sweep_lv<G, T, UPPER><<<grid, kLvThreads, 0, e->stream>>>(
+        nlev, off.get(), desc.get(), perm.get(), e->rp.get(), e->ci.get(), e->dg.get(), val, e->nnz, b, out, reset,
+        gate, e->lvprog.get(), epoch, reinterpret_cast<int*>(e->lvprog.get() + 1), watchdog_ticks(kWatchdogSeconds));
 }
 
 template <typename T, bool UPPER>
