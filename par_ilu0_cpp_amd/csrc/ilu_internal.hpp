@@ -75,8 +75,10 @@ struct lljbash::IluSolver::Ext {
     // on first use for (pattern_version, slots per level)
     pilu::DevBuf<int4> lvdesc_l, lvdesc_u;
     int lvkey_l[2] = {-1, 0}, lvkey_u[2] = {-1, 0};
-    pilu::DevBuf<unsigned long long> lvprog;  // [0]: progress word of the mode-4 sweep; [1]: sink
+    pilu::DevBuf<unsigned long long> lvprog;  // [0]: progress word of the mode-4 sweep; [1]: sink; [2]: slot check
     unsigned lv_epoch = 0;
+    pilu::DevBuf<int> lvbad;  // per level: 1 = may not use mode 4's LDS cache (scratch of the descriptor build)
+    int lvcached_l = 0, lvcached_u = 0;  // levels that use mode 4's LDS cache
     int pattern_version = 0;
 
     pilu::DevBuf<double> vin;    // values to factorize (scratch)

@@ -393,6 +393,32 @@ __global__ __launch_bounds__(512) void k_sweep_blk(int n, int blk, int nblk, con
 // ---------------------------------------------------------------------
 constexpr int kLvThreads = 1024;
 constexpr int kLvH = 4;
+// With `cache` set, the x of recently solved rows are also kept in LDS, in
+// slot row % kLvCache tagged with the row (96 KB of dynamic LDS); a read
+// whose slot holds another row goes to memory.  A level uses the ring only
+// if no slot is both written and read, or written twice, within it
+// (k_lv_slots; bit 30 of the descriptor's row count marks the others, whose
+// rows are then neither read from nor written to the ring), so a slot's
+// value and tag always come from the same row.
+constexpr int kLvCache = 8192;
+constexpr size_t kLvLds = kLvCache * (sizeof(double) + sizeof(int));
+
+__device__ __forceinline__ double lv_x(int c, const double* out, const double* xs, const int* xt, bool cache) {
+    if (cache) {
+        const int sl = c & (kLvCache - 1);
+        if (xt[sl] == c) return xs[sl];
+    }
+    return out[c];
+}
+
+__device__ __forceinline__ void lv_put(int row, double v, double* out, double* xs, int* xt, bool cache) {
+    out[row] = v;
+    if (cache) {
+        const int sl = row & (kLvCache - 1);
+        xs[sl] = v;
+        xt[sl] = row;
+    }
+}
 
 template <typename T>
 struct LvRow {
@@ -435,14 +461,17 @@ __device__ __forceinline__ void lv_level(int l, int nlev, LvRow<T>& r, int4& d, 
                                          const int4* __restrict__ desc, const int* __restrict__ perm,
                                          const int* __restrict__ rp, const int* __restrict__ ci,
                                          const int* __restrict__ dg, const T* __restrict__ val, long last,
-                                         const double* b, double* out, double* reset) {
+                                         const double* b, double* out, double* reset, double* xs, int* xt,
+                                         bool cache) {
     constexpr int kSlots = kLvThreads / G;
     const int j = static_cast<int>(threadIdx.x) / G;
     const int t = threadIdx.x & (G - 1);
     const LvRow<T> cur = r;
+    const int width = cur.w & 0x3fffffff;
+    cache = cache && (cur.w >> 30) == 0;
     double x[kLvH];
 #pragma unroll
-    for (int q = 0; q < kLvH; ++q) x[q] = out[cur.c[q]];
+    for (int q = 0; q < kLvH; ++q) x[q] = lv_x(cur.c[q], out, xs, xt, cache);
     lv_fetch<G, T, UPPER>(d, t, last, ci, val, b, r);
     {
         const int l4 = l + 4 < nlev ? l + 4 : nlev - 1;
@@ -452,23 +481,25 @@ __device__ __forceinline__ void lv_level(int l, int nlev, LvRow<T>& r, int4& d, 
 #pragma unroll
     for (int q = 0; q < kLvH; ++q)
         if (cur.e + q * G < cur.end) acc = fma(-x[q], static_cast<double>(cur.v[q]), acc);
-    for (int e = cur.e + kLvH * G; e < cur.end; e += G) acc = fma(-out[ci[e]], static_cast<double>(val[e]), acc);
+    for (int e = cur.e + kLvH * G; e < cur.end; e += G)
+        acc = fma(-lv_x(ci[e], out, xs, xt, cache), static_cast<double>(val[e]), acc);
     const double s = group_sum<G>(acc);
     if (t == 0 && cur.row >= 0) {
-        out[cur.row] = UPPER ? s / static_cast<double>(cur.dv) : s;
+        lv_put(cur.row, UPPER ? s / static_cast<double>(cur.dv) : s, out, xs, xt, cache);
         if (reset) reinterpret_cast<uint64_t*>(reset)[cur.row] = kPending;
     }
-    if (cur.w > kSlots) {
-        const int p0 = lev_off[l], p1 = p0 + cur.w;
+    if (width > kSlots) {
+        const int p0 = lev_off[l], p1 = p0 + width;
         for (int p = p0 + kSlots + j; p < p1; p += kSlots) {
             const int row = perm[p];
             const int dd = dg[row];
             const int e0 = UPPER ? dd + 1 : rp[row], end = UPPER ? rp[row + 1] : dd;
             double a = t == 0 ? b[row] : 0.0;
-            for (int e = e0 + t; e < end; e += G) a = fma(-out[ci[e]], static_cast<double>(val[e]), a);
+            for (int e = e0 + t; e < end; e += G)
+                a = fma(-lv_x(ci[e], out, xs, xt, cache), static_cast<double>(val[e]), a);
             const double sm = group_sum<G>(a);
             if (t == 0) {
-                out[row] = UPPER ? sm / static_cast<double>(val[dd]) : sm;
+                lv_put(row, UPPER ? sm / static_cast<double>(val[dd]) : sm, out, xs, xt, cache);
                 if (reset) reinterpret_cast<uint64_t*>(reset)[row] = kPending;
             }
         }
@@ -547,7 +578,8 @@ __global__ __launch_bounds__(kLvThreads) void k_sweep_lv(int nlev, const int* __
                                                          const int* __restrict__ dg, const T* __restrict__ val,
                                                          long nnz, const double* b, double* out, double* reset,
                                                          const int* gate, unsigned long long* prog, unsigned epoch,
-                                                         int* sink, uint64_t wd_limit) {
+                                                         int* sink, uint64_t wd_limit, int cache) {
+    extern __shared__ double lv_lds[];
     if (gate && *gate == 0) return;
     if (nlev <= 0 || nnz <= 0) return;
     if (blockIdx.x != 0) {
@@ -560,6 +592,13 @@ __global__ __launch_bounds__(kLvThreads) void k_sweep_lv(int nlev, const int* __
     const int j = static_cast<int>(threadIdx.x) / G;
     const int t = threadIdx.x & (G - 1);
     const long last = nnz - 1;
+    double* xs = lv_lds;
+    int* xt = reinterpret_cast<int*>(lv_lds + kLvCache);
+    const bool use = cache != 0;
+    if (use) {
+        for (int i = threadIdx.x; i < kLvCache; i += kLvThreads) xt[i] = -1;
+        __syncthreads();
+    }
     auto dsc = [&](int l) { return desc[static_cast<long>(l < nlev ? l : nlev - 1) * kSlots + j]; };
     LvRow<T> r0, r1;
     lv_fetch<G, T, UPPER>(dsc(0), t, last, ci, val, b, r0);
@@ -568,28 +607,30 @@ __global__ __launch_bounds__(kLvThreads) void k_sweep_lv(int nlev, const int* __
     const unsigned long long ep = static_cast<unsigned long long>(epoch) << 32;
     for (int l = 0; l < nlev; l += 4) {
         if (threadIdx.x == 0) __hip_atomic_store(prog, ep | static_cast<unsigned>(l), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lv_level<G, T, UPPER>(l, nlev, r0, d0, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+        lv_level<G, T, UPPER>(l, nlev, r0, d0, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset, xs, xt, use);
         if (l + 1 >= nlev) break;
-        lv_level<G, T, UPPER>(l + 1, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+        lv_level<G, T, UPPER>(l + 1, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset, xs, xt, use);
         if (l + 2 >= nlev) break;
-        lv_level<G, T, UPPER>(l + 2, nlev, r0, d0, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+        lv_level<G, T, UPPER>(l + 2, nlev, r0, d0, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset, xs, xt, use);
         if (l + 3 >= nlev) break;
-        lv_level<G, T, UPPER>(l + 3, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset);
+        lv_level<G, T, UPPER>(l + 3, nlev, r1, d1, lev_off, desc, perm, rp, ci, dg, val, last, b, out, reset, xs, xt, use);
     }
     if (threadIdx.x == 0) __hip_atomic_store(prog, ep | static_cast<unsigned>(nlev), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // desc[l * slots + j] of k_sweep_lv: the j-th row of level l, its first
 // entry and end in the sweep's direction (row -1, no entries, past the
-// level's last row), and the level's row count.
+// level's last row), and the level's row count, with bit 30 set when the
+// level may not use the LDS cache (bad[l]).
 template <bool UPPER>
 __global__ void k_lv_desc(int nlev, int slots, const int* __restrict__ lev_off, const int* __restrict__ perm,
-                          const int* __restrict__ rp, const int* __restrict__ dg, int4* __restrict__ desc) {
+                          const int* __restrict__ rp, const int* __restrict__ dg, const int* __restrict__ bad,
+                          int4* __restrict__ desc) {
     const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= static_cast<long>(nlev) * slots) return;
     const int l = static_cast<int>(i / slots), j = static_cast<int>(i % slots);
     const int p = lev_off[l] + j;
-    int4 d = make_int4(-1, 0, 0, lev_off[l + 1] - lev_off[l]);
+    int4 d = make_int4(-1, 0, 0, (lev_off[l + 1] - lev_off[l]) | (bad[l] ? 1 << 30 : 0));
     if (p < lev_off[l + 1]) {
         const int row = perm[p];
         d.x = row;
@@ -597,6 +638,47 @@ __global__ void k_lv_desc(int nlev, int slots, const int* __restrict__ lev_off, 
         d.z = UPPER ? rp[row + 1] : dg[row];
     }
     desc[i] = d;
+}
+
+// lev_bad[l] = 1 unless the rows of level l have distinct k_sweep_lv cache
+// slots and none is the slot of a dependency of a row of the level (the
+// slots written during such a level are exactly its rows', none of them
+// read during it, so a read never meets a half-written slot); nbad[0]
+// counts the others.  One workgroup per level (grid = nlev) marks its rows'
+// slots in an LDS bitmap, then tests every dependency's slot against it.
+template <bool UPPER>
+__global__ __launch_bounds__(256) void k_lv_slots(const int* __restrict__ lev_off, const int* __restrict__ perm,
+                                                  const int* __restrict__ rp, const int* __restrict__ dg,
+                                                  const int* __restrict__ ci, int* __restrict__ lev_bad, int* nbad) {
+    constexpr int M = kLvCache - 1;
+    __shared__ unsigned bits[kLvCache / 32];
+    const int l = blockIdx.x;
+    for (int i = threadIdx.x; i < kLvCache / 32; i += 256) bits[i] = 0u;
+    __syncthreads();
+    const int p0 = lev_off[l], p1 = lev_off[l + 1];
+    bool bad = false;
+    for (int p = p0 + static_cast<int>(threadIdx.x); p < p1; p += 256) {
+        const int sl = perm[p] & M;
+        const unsigned bit = 1u << (sl & 31);
+        if (atomicOr(&bits[sl >> 5], bit) & bit) bad = true;
+    }
+    __syncthreads();
+    for (int p = p0 + static_cast<int>(threadIdx.x); p < p1 && !bad; p += 256) {
+        const int row = perm[p];
+        const int e0 = UPPER ? dg[row] + 1 : rp[row], e1 = UPPER ? rp[row + 1] : dg[row];
+        for (int e = e0; e < e1; ++e) {
+            const int sc = ci[e] & M;
+            if ((bits[sc >> 5] >> (sc & 31)) & 1u) {
+                bad = true;
+                break;
+            }
+        }
+    }
+    const bool any = __syncthreads_or(bad ? 1 : 0) != 0;
+    if (threadIdx.x == 0) {
+        lev_bad[l] = any ? 1 : 0;
+        if (any) atomicAdd(nbad, 1);
+    }
 }
 
 // off[l] = first position of level l in a level-sorted order (every level

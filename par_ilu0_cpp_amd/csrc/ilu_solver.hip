@@ -857,25 +857,37 @@ void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* re
     const long nd = static_cast<long>(nlev) * slots;
     if (off.size() < static_cast<size_t>(nlev) + 1 || perm.size() < static_cast<size_t>(e->n))
         throw std::runtime_error("level-walk sweep: level schedule missing");
+    if (e->lvprog.size() < 3) {
+        e->lvprog.reserve(3);
+        PILU_HIP(hipMemset(e->lvprog.get(), 0, 3 * sizeof(unsigned long long)));
+    }
+    int& cached = UPPER ? e->lvcached_u : e->lvcached_l;
     if (key[0] != e->pattern_version || key[1] != slots) {
+        int* nbad = reinterpret_cast<int*>(e->lvprog.get() + 2);
+        e->lvbad.reserve(static_cast<size_t>(nlev));
+        PILU_HIP(hipMemsetAsync(nbad, 0, sizeof(int), e->stream));
+        k_lv_slots<UPPER><<<nlev, 256, 0, e->stream>>>(off.get(), perm.get(), e->rp.get(), e->dg.get(), e->ci.get(),
+                                                      e->lvbad.get(), nbad);
+        PILU_HIP(hipGetLastError());
         desc.reserve(static_cast<size_t>(nd));
         k_lv_desc<UPPER><<<grid_for(nd, 256), 256, 0, e->stream>>>(nlev, slots, off.get(), perm.get(), e->rp.get(),
-                                                                  e->dg.get(), desc.get());
+                                                                  e->dg.get(), e->lvbad.get(), desc.get());
         PILU_HIP(hipGetLastError());
+        cached = nlev - read_int(nbad, e->stream);
         key[0] = e->pattern_version;
         key[1] = slots;
-    }
-    if (e->lvprog.size() < 2) {
-        e->lvprog.reserve(2);
-        PILU_HIP(hipMemset(e->lvprog.get(), 0, 2 * sizeof(unsigned long long)));
     }
     // workgroup 0 solves; workgroups 8, 16, ..., 8 * kLvHelpers (on its XCD)
     // warm its L2; the others return at once
     const int grid = 1 + 8 * kLvHelpers;
     const unsigned epoch = ++e->lv_epoch;
-    k_sweep_lv<G, T, UPPER><<<grid, kLvThreads, 0, e->stream>>>(
-        nlev, off.get(), desc.get(), perm.get(), e->rp.get(), e->ci.get(), e->dg.get(), val, e->nnz, b, out, reset,
-        gate, e->lvprog.get(), epoch, reinterpret_cast<int*>(e->lvprog.get() + 1), watchdog_ticks(kWatchdogSeconds));
+    auto kern = k_sweep_lv<G, T, UPPER>;
+    PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(kLvLds)));
+    kern<<<grid, kLvThreads, kLvLds, e->stream>>>(nlev, off.get(), desc.get(), perm.get(), e->rp.get(), e->ci.get(),
+                                                 e->dg.get(), val, e->nnz, b, out, reset, gate, e->lvprog.get(), epoch,
+                                                 reinterpret_cast<int*>(e->lvprog.get() + 1),
+                                                 watchdog_ticks(kWatchdogSeconds), cached > 0 ? 1 : 0);
 }
 
 template <typename T, bool UPPER>
@@ -1359,6 +1371,7 @@ std::string IluSolver::DescribeSchedule() const {
     std::string js(buf);
     const bool live_l = e->nblk > 0 && e->pack_l.version == e->part_version;
     const bool live_u = e->nblk > 0 && e->pack_u.version == e->part_version;
+    js += "\"lv_cached_levels\": [" + std::to_string(e->lvcached_l) + ", " + std::to_string(e->lvcached_u) + "], ";
     js += "\"pack_l\": " + (live_l ? pack(e->pack_l) : std::string("null"));
     js += ", \"pack_u\": " + (live_u ? pack(e->pack_u) : std::string("null")) + "}";
     return js;

@@ -462,3 +462,38 @@ def test_autotune_tries_level_walk_on_deep_patterns(oracle):
     b = np.random.default_rng(1).uniform(-1, 1, rp.size - 1)
     xo = oracle.substitute(rp, ci, dg, ofac, b)
     assert np.max(np.abs(ilu.Substitute(b) - xo)) <= 1e-12 * np.max(np.abs(xo))
+
+
+@pytest.mark.parametrize("case", ["poisson128", "two_copies"])
+def test_level_walk_cache_paths_bit_exact(oracle, case):
+    """Mode 4 keeps recent x in an LDS ring of 8192 tagged slots.  2D Poisson
+    128^2 (n = 16384): the ring is used and reads of rows whose slot was
+    reused go to memory.  Two copies of an 8192-row tridiagonal system side
+    by side: rows i and i + 8192 share a level and a slot, so no level uses
+    the ring.
+    Both bit-identical to the oracle's lane-order sweep, applied twice."""
+    if case == "poisson128":
+        a = gen.poisson2d(128)
+        rp, ci, va = a.row_ptr, a.col_idx, a.value
+    else:
+        import scipy.sparse as sp
+        m = sp.diags([-1.0, 2.5, -1.0], [-1, 0, 1], shape=(8192, 8192), format="csr")
+        a2 = sp.block_diag([m, m], format="csr")
+        a2.sort_indices()
+        rp = a2.indptr.astype(np.int32)
+        ci = a2.indices.astype(np.int32)
+        va = a2.data.astype(np.float64)
+    rp, ci, va = (np.ascontiguousarray(v) for v in (rp, ci, va))
+    n = rp.size - 1
+    ilu = factored(rp, ci, va)
+    ofac, dg, _ = oracle.ilu_factorize(rp, ci, va)
+    rng = np.random.default_rng(n)
+    for g in (4, 16, 64):
+        ilu.SetSweepLanes(g, g)
+        ilu.SetSweepMode(4, 4, 0)
+        for _ in range(2):
+            b = rng.uniform(-1, 1, n)
+            assert np.array_equal(ilu.Substitute(b), oracle.substitute_lanes(rp, ci, dg, ofac, b, g)), g
+        sch = ilu.Schedule()
+        nl, nu = ilu.Levels()
+        assert sch["lv_cached_levels"] == ([nl, nu] if case == "poisson128" else [0, 0])
