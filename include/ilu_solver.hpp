@@ -80,7 +80,9 @@ public:
     // a packed, schedule-ordered copy of the factors (refreshed at every
     // Factorize) with its loads issued several passes ahead and a workgroup
     // barrier per local level, 3 the same with the block's x held in LDS as
-    // value-flags instead of barriers.
+    // value-flags instead of barriers, 4 one workgroup walking the level
+    // schedule with a barrier per level (no hand-offs between workgroups;
+    // for deep schedules with few rows per level).
     void SetSweepMode(int mode_l, int mode_u, int blk_rows);
     // Declares that row x + nx (y + ny z) is grid cell (x, y, z); enables
     // tiled partitions of the block-local sweeps.

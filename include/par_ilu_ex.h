@@ -34,7 +34,8 @@ int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
 int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
 
 /* Autotune the sweeps after Factorize: level-scheduled rows with 1..64
- * lanes per row and the block-local sweeps (contiguous blocks of
+ * lanes per row, the single-workgroup level walk with 4..64 lanes (where
+ * levels average at most 256 rows) and the block-local sweeps (contiguous blocks of
  * 4096/8192/16384 rows, and, once the grid shape is declared, tiles spanning
  * the x extent (up to 128 cells) and several planes; CSR factors, or packed
  * factors with level barriers or LDS value-flags) with 2..32 lanes are each
@@ -47,7 +48,10 @@ int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap);
 /* Explicit sweep algorithm per direction (0 level rows, 1 block-local over
  * contiguous blocks of blk_rows rows, 2 block-local over packed factors,
  * software-pipelined, with a workgroup barrier per local level, 3 the same
- * with the block's x as LDS value-flags and no barriers) and the current
+ * with the block's x as LDS value-flags and no barriers, 4 one workgroup
+ * walking the level schedule with a barrier per level, for deep schedules
+ * with few rows per level; blk_rows is ignored by modes 0 and 4) and the
+ * current
  * configuration (cfg[8]: mode_l,
  * lanes_l, mode_u, lanes_u, blk_rows, tile x, tile y, tile z). */
 #define IluSolverSetSweepModeEx LLJBASH_DECL(IluSolverSetSweepModeEx)
