@@ -73,7 +73,8 @@ def parse():
     return p.parse_args()
 
 
-SWEEP_KERNELS = {0: "k_sweep", 1: "k_sweep_blk", 2: "k_sweep_pp (level barriers)", 3: "k_sweep_pp (LDS value-flags)"}
+SWEEP_KERNELS = {0: "k_sweep", 1: "k_sweep_blk", 2: "k_sweep_pp (level barriers)", 3: "k_sweep_pp (LDS value-flags)",
+                 4: "k_sweep_lv (one workgroup, level barriers)"}
 
 
 def copy_calibration():
