@@ -185,11 +185,11 @@ class IluSolver:
         """Times every sweep shape and keeps the fastest per direction.
         -> (config dict, table[rows, 8]: mode, block rows, tile x, y, z,
         lanes, fwd ms, bwd ms)."""
-        table = np.zeros((96, 8), dtype=np.float64)
-        rc = fn("IluSolverAutotuneEx")(self.hdl, int(reps), _ptr(table, C.c_double), 96)
+        table = np.zeros((128, 8), dtype=np.float64)
+        rc = fn("IluSolverAutotuneEx")(self.hdl, int(reps), _ptr(table, C.c_double), 128)
         if rc < 0:
             raise RuntimeError("IluSolverAutotuneEx failed")
-        return self.SweepConfig(), table[: min(rc, 96)].copy()
+        return self.SweepConfig(), table[: min(rc, 128)].copy()
 
     def SetSweepMode(self, mode_l: int, mode_u: int, blk_rows: int = 0) -> None:
         if fn("IluSolverSetSweepModeEx")(self.hdl, int(mode_l), int(mode_u), int(blk_rows)) != 0:
