@@ -61,6 +61,12 @@ public:
     // Lanes cooperating on one row in the L and U sweeps (1..64, power of
     // two); 0 picks a value from the average row length.
     void SetSweepLanes(int lanes_l, int lanes_u);
+    // Wall-clock limit (seconds, 0..3600; default 10) of every bounded
+    // dependency wait in the analysis, factorization and sweep kernels; the
+    // symbolic pass waits up to twice as long.  A wait past the limit gives
+    // up, the kernel drains and the call throws; a solver whose analysis or
+    // factorization timed out refuses further sweeps until it is redone.
+    void SetWatchdog(double seconds);
     // Per-row diagonal compensation used by the next factorizations:
     // d_omega (device, n floats, caller-owned; null turns it off).  Row i
     // adds omega[i] times the sum of its dropped fill updates to its pivot.

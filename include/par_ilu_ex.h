@@ -33,6 +33,14 @@ int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
 #define IluSolverSetDiagonalCompensationEx LLJBASH_DECL(IluSolverSetDiagonalCompensationEx)
 int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
 
+/* Wall-clock limit in seconds (0..3600, default 10) of every bounded
+ * dependency wait in the analysis, factorization and sweep kernels; the
+ * symbolic pass waits up to twice as long.  A wait past it makes the call
+ * fail (-1) after every kernel has drained; the solver then refuses sweeps
+ * until Setup / Factorize complete again. */
+#define IluSolverSetWatchdogEx LLJBASH_DECL(IluSolverSetWatchdogEx)
+int IluSolverSetWatchdogEx(IluSolverHdl hdl, double seconds);
+
 /* Autotune the sweeps after Factorize: level-scheduled rows with 1..64
  * lanes per row, the single-workgroup level walk with 4..64 lanes (where
  * levels average at most 256 rows) and the block-local sweeps (contiguous blocks of

@@ -181,6 +181,11 @@ class IluSolver:
         if fn("IluSolverSetDiagonalCompensationEx")(self.hdl, C.c_void_p(d_omega or None)) != 0:
             raise RuntimeError("SetDiagonalCompensation failed")
 
+    def SetWatchdog(self, seconds: float) -> None:
+        """Wall-clock limit of every bounded dependency wait (0..3600 s)."""
+        if fn("IluSolverSetWatchdogEx")(self.hdl, float(seconds)) != 0:
+            raise RuntimeError("SetWatchdog failed")
+
     def Autotune(self, reps: int = 3):
         """Times every sweep shape and keeps the fastest per direction.
         -> (config dict, table[rows, 8]: mode, block rows, tile x, y, z,

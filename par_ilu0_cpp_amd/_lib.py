@@ -86,6 +86,7 @@ SIGNATURES = [
     ("IluSolverSetSweepLanesEx", _int, [_vp, _int, _int]),
     ("IluSolverGetLevelsEx", _int, [_vp, P(_int), P(_int)]),
     ("IluSolverScheduleJsonEx", _int, [_vp, C.c_char_p, _int]),
+    ("IluSolverSetWatchdogEx", _int, [_vp, _dbl]),
     ("IluSolverAutotuneEx", _int, [_vp, _int, P(_dbl), _int]),
     ("IluSolverSetSweepModeEx", _int, [_vp, _int, _int, _int]),
     ("IluSolverSetGridShapeEx", _int, [_vp, _int, _int, _int]),

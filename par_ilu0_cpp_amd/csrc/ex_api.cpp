@@ -44,6 +44,10 @@ int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega) {
     return guarded("SetDiagonalCompensation", [&] { S(hdl)->SetDiagonalCompensation(d_omega); return 0; });
 }
 
+int IluSolverSetWatchdogEx(IluSolverHdl hdl, double seconds) {
+    return guarded("SetWatchdog", [&] { S(hdl)->SetWatchdog(seconds); return 0; });
+}
+
 int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap) {
     return guarded("Autotune", [&] { return S(hdl)->AutotuneSweeps(reps, table, cap); });
 }

@@ -95,6 +95,10 @@ struct lljbash::IluSolver::Ext {
     const float* omega_ptr = nullptr;  // per-row compensation (device, caller-owned)
 
     int precision = 64;
+    // Wall-clock limit of every bounded dependency wait of the analysis,
+    // factorization and sweep kernels (the symbolic pass gets twice this).
+    // A wait past it gives up: the kernel drains and the call throws.
+    double watchdog_s = 10.0;
     int lanes_l = 0, lanes_u = 0;  // 0 = heuristic
     // Sweep algorithm per direction: 0 = level-scheduled rows (k_sweep),
     // 1 = block-local over contiguous blocks of blk_rows rows (k_sweep_blk),

@@ -50,7 +50,6 @@ uint64_t watchdog_ticks(double seconds) {
 // Error-word slots of IluSolver::Ext::err.
 enum ErrSlot { kErrFactor = 0, kErrSweep = 1, kErrSorted = 2, kErrDiag = 3, kErrMaxLev = 4, kErrRowPtr = 5,
                kErrLevels = 6 };
-constexpr double kWatchdogSeconds = 10.0;
 
 void check_sweep_errors(lljbash::IluSolver::Ext* e) {
     int v = 0;
@@ -143,7 +142,7 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm, DevBuf<int>& lev_off) 
     int* level = e->level.get();
     int* maxlev = e->err.get() + kErrMaxLev;
     int* ctr = e->ctr.get();
-    const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
+    const uint64_t wd = watchdog_ticks(e->watchdog_s);
     k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
     PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
     (void)ctr;
@@ -195,7 +194,7 @@ void build_block_schedule(Ext* e, bool upper, const int* block_of, int nblk, con
     int* maxlev = e->err.get() + kErrMaxLev;
     k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
     PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
-    const uint64_t wd = watchdog_ticks(kWatchdogSeconds);
+    const uint64_t wd = watchdog_ticks(e->watchdog_s);
     const int* bo = block_of;
     if (upper) launch_levels<true>(e, bo, level, maxlev, wd);
     else launch_levels<false>(e, bo, level, maxlev, wd);
@@ -739,7 +738,7 @@ void sweep_one(Ext* e, const T* val, const double* b, double* out, double* reset
     k_sweep<G, T, UPPER><<<grid, kBlock, 0, e->stream>>>(e->n, UPPER ? e->perm_u.get() : e->perm_l.get(),
                                                         e->rp.get(), e->ci.get(), e->dg.get(), val, b, out,
                                                         reset, gate, ctr, e->err.get() + kErrSweep,
-                                                        watchdog_ticks(kWatchdogSeconds));
+                                                        watchdog_ticks(e->watchdog_s));
 }
 
 template <int G, typename T, bool UPPER>
@@ -752,7 +751,7 @@ void sweep_blk_one(Ext* e, const T* val, const double* b, double* out, double* r
                                           UPPER ? e->blev_u.get() : e->blev_l.get(),
                                           UPPER ? e->loff_u.get() : e->loff_l.get(), e->rp.get(), e->ci.get(),
                                           e->dg.get(), val, b, out, reset, gate, ctr, e->err.get() + kErrSweep,
-                                          watchdog_ticks(kWatchdogSeconds));
+                                          watchdog_ticks(e->watchdog_s));
 }
 
 template <typename T, bool UPPER>
@@ -807,7 +806,7 @@ void sweep_pp_one(Ext* e, const T* pval, const T* pdiag, const double* b, double
                                           reinterpret_cast<const PassDesc*>(pk.pdesc.get()),
                                           reinterpret_cast<const int2*>(pk.rowinfo.get()), pk.pcol.get(), pval, pdiag,
                                           b, out, reset, gate, ctr, e->err.get() + kErrSweep,
-                                          watchdog_ticks(kWatchdogSeconds), e->max_block_rows);
+                                          watchdog_ticks(e->watchdog_s), e->max_block_rows);
 }
 
 template <int G, typename T, bool UPPER>
@@ -845,6 +844,10 @@ void sweep_pp_dispatch(int lanes, Ext* e, const double* b, double* out, double* 
     }
 }
 
+// Most (level, row slot) descriptors the level walk may allocate per
+// direction: 32 per row (a 16-byte descriptor each), at least 2^20.
+long lv_desc_cap(long n) { return std::max(32L * n, 1L << 20); }
+
 template <int G, typename T, bool UPPER>
 void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* reset, const int* gate) {
     constexpr int slots = kLvThreads / G;
@@ -857,6 +860,9 @@ void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* re
     const long nd = static_cast<long>(nlev) * slots;
     if (off.size() < static_cast<size_t>(nlev) + 1 || perm.size() < static_cast<size_t>(e->n))
         throw std::runtime_error("level-walk sweep: level schedule missing");
+    if (nd > lv_desc_cap(e->n))
+        throw std::runtime_error("level-walk sweep: levels x row slots exceed the descriptor budget "
+                                 "(use fewer lanes per row or another sweep mode)");
     if (e->lvprog.size() < 3) {
         e->lvprog.reserve(3);
         PILU_HIP(hipMemset(e->lvprog.get(), 0, 3 * sizeof(unsigned long long)));
@@ -887,7 +893,7 @@ void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* re
     kern<<<grid, kLvThreads, kLvLds, e->stream>>>(nlev, off.get(), desc.get(), perm.get(), e->rp.get(), e->ci.get(),
                                                  e->dg.get(), val, e->nnz, b, out, reset, gate, e->lvprog.get(), epoch,
                                                  reinterpret_cast<int*>(e->lvprog.get() + 1),
-                                                 watchdog_ticks(kWatchdogSeconds), cached > 0 ? 1 : 0);
+                                                 watchdog_ticks(e->watchdog_s), cached > 0 ? 1 : 0);
 }
 
 template <typename T, bool UPPER>
@@ -940,7 +946,7 @@ void factor_short_one(Ext* e, float* f32) {
     k_factor_short<G, CAP, P><<<grid, kBlock, 0, e->stream>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
                                                              e->dg.get(), e->vin.get(), e->fac.get(), f32,
                                                              e->err.get() + kErrFactor, e->ctr.get() + 2,
-                                                             watchdog_ticks(kWatchdogSeconds), e->omega_ptr);
+                                                             watchdog_ticks(e->watchdog_s), e->omega_ptr);
 }
 
 template <int G, int CAP>
@@ -950,12 +956,15 @@ void factor_one(Ext* e, float* f32) {
     k_factor<G, CAP><<<grid, kBlock, 0, e->stream>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
                                                     e->dg.get(), e->vin.get(), e->fac.get(), f32,
                                                     e->err.get() + kErrFactor, e->ctr.get() + 2,
-                                                    watchdog_ticks(kWatchdogSeconds), e->omega_ptr);
+                                                    watchdog_ticks(e->watchdog_s), e->omega_ptr);
 }
 
 // Runs the factorization kernel on e->vin; returns false on a zero pivot.
 bool run_factor(Ext* e) {
     hipStream_t s = e->stream;
+    // the factors and packs are overwritten from here on: until this run
+    // completes, no sweep may use them
+    e->factored = false;
     float* f32 = nullptr;
     if (e->precision == 32) {
         e->fac32.reserve(e->nnz);
@@ -1104,7 +1113,8 @@ int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx,
     Ext* e = ext_;
     e->analysed = false;  // the previous pattern is gone whether or not this completes
     pilu::SymbolicResult res;
-    const int nnz_f = pilu::symbolic_device(n, d_row_ptr, d_col_idx, nullptr, lof, res, e->stream);
+    const int nnz_f = pilu::symbolic_device(n, d_row_ptr, d_col_idx, nullptr, lof, res, e->stream,
+                                            2.0 * e->watchdog_s);
     if (nnz_f < 0) return -1;
     int nnz_a = 0;
     PILU_HIP(hipMemcpy(&nnz_a, d_row_ptr + n, sizeof(int), hipMemcpyDeviceToHost));
@@ -1149,6 +1159,12 @@ void IluSolver::Substitute(const double* b, double* x) {
 
 void IluSolver::SubstituteDevice(const double* d_b, double* d_x) {
     pilu::ilu_apply(pilu::need_setup(ext_), d_b, d_x, nullptr);
+}
+
+void IluSolver::SetWatchdog(double seconds) {
+    if (!(seconds >= 0.0) || seconds > 3600.0) throw std::runtime_error("watchdog limit must be in [0, 3600] s");
+    if (!ext_) ext_ = new Ext;
+    ext_->watchdog_s = seconds;
 }
 
 void IluSolver::SetDiagonalCompensation(const float* d_omega) {
@@ -1252,6 +1268,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     if (e->n / deep <= kLvTryRows) {
         e->mode_l = e->mode_u = 4;
         for (int g : {4, 8, 16, 32, 64}) {
+            if (static_cast<long>(deep) * (pilu::kLvThreads / g) > pilu::lv_desc_cap(e->n)) continue;
             double tl, tu;
             pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
             record(4, Part{0, 0, 0, 0}, g, tl, tu);

@@ -278,7 +278,7 @@ void check(hipError_t e, const char* what) {
 }  // namespace
 
 int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_aval, int lof, SymbolicResult& out,
-                    hipStream_t s) {
+                    hipStream_t s, double watchdog_s) {
     if (n <= 0) return -1;
     if (n >= (1 << 27) - 1) throw std::runtime_error("IlukSymbolic: n must be below 2^27 - 1");
     if (lof < 0 || lof > 31) throw std::runtime_error("IlukSymbolic: level of fill must be in [0, 31]");
@@ -307,11 +307,11 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
         if (short_rows)
             k_symbolic<kSymCapShort><<<grid, kSymWaves * 64, 0, s>>>(
                 n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
-                misc.get() + 2, misc.get(), watchdog_ticks(20.0));
+                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s));
         else
             k_symbolic<kSymCapLong><<<grid, kSymWaves * 64, 0, s>>>(
                 n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
-                misc.get() + 2, misc.get(), watchdog_ticks(20.0));
+                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s));
         check(hipGetLastError(), "k_symbolic");
         int err = 0;
         check(hipMemcpyAsync(&err, misc.get() + 2, sizeof(int), hipMemcpyDeviceToHost, s), "err");

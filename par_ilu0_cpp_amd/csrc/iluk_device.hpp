@@ -18,8 +18,9 @@ struct SymbolicResult {
 
 // Runs the pass on device arrays of A (values optional).  Returns nnz(F),
 // or -1 for a row of A with a repeated column; throws std::runtime_error on
-// device errors or capacity limits.
+// device errors, capacity limits or a dependency wait longer than
+// watchdog_s seconds.
 int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_aval, int lof, SymbolicResult& out,
-                    hipStream_t s);
+                    hipStream_t s, double watchdog_s = 20.0);
 
 }  // namespace pilu

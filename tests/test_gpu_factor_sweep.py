@@ -308,18 +308,23 @@ def test_schedule_levels_closed_form(oracle, lof, closed_form):
     assert sch["pack_l"]["entries"] >= (o[0][-1] - a.n) // 2 and sch["pack_l"]["max_passes_per_block"] % 8 == 0
 
 
-@pytest.mark.parametrize("tile", [(8, 8, 4), (20, 5, 3), (7, 9, 11), (20, 20, 1)])
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("tile", [(8, 8, 4), (20, 5, 3), (7, 9, 11), (20, 20, 1), (20, 4, 8)])
 @pytest.mark.parametrize("lof", [0, 1])
-def test_tiled_sweeps_bit_exact(oracle, tile, lof):
+def test_tiled_sweeps_bit_exact(oracle, tile, lof, bits):
     """Block-local sweeps over 3D tiles of the grid (including tiles that do
-    not divide it) reproduce the oracle's lane-order sweep bit for bit, in
-    either direction and mixed with the level-scheduled sweep."""
+    not divide it, and the benchmark's x-extent x 4 x 8 shape) reproduce the
+    oracle's lane-order sweep bit for bit, in either direction and mixed
+    with the level-scheduled sweep, with FP64 factors and with FP32 factors
+    (the oracle then sweeps the same float-rounded factors)."""
     s = 20
     a = gen.convdiff3d(s)
     o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, lof)
     frp, fci = o[0], o[1]
-    ilu = factored(frp, fci, o[3])
+    ilu = factored(frp, fci, o[3], bits)
     ofac, dg, _ = oracle.ilu_factorize(frp, fci, o[3])
+    if bits == 32:
+        ofac = ofac.astype(np.float32).astype(np.float64)
     ilu.SetGridShape(s, s, s)
     ilu.SetSweepTiles(2, 2, *tile)
     sch = ilu.Schedule()
@@ -497,3 +502,63 @@ def test_level_walk_cache_paths_bit_exact(oracle, case):
         sch = ilu.Schedule()
         nl, nu = ilu.Levels()
         assert sch["lv_cached_levels"] == ([nl, nu] if case == "poisson128" else [0, 0])
+
+
+def test_watchdog_zero_refuses_work(oracle):
+    """A watchdog limit of 0 makes every bounded dependency wait give up at
+    once: the kernels drain normally and the call that launched them fails.
+    A solver whose factorization or analysis did not complete refuses to
+    sweep or solve, and works again once both are redone with a limit."""
+    a = gen.convdiff3d(20)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)
+    frp, fci, fval = o[0], o[1], o[3]
+    ilu = factored(frp, fci, fval)
+    b = np.random.default_rng(12).uniform(-1, 1, a.n)
+    ofac, dg, _ = oracle.ilu_factorize(frp, fci, fval)
+    x_ok = ilu.Substitute(b)
+    assert np.max(np.abs(x_ok - oracle.substitute(frp, fci, dg, ofac, b))) <= 1e-12 * np.max(np.abs(x_ok))
+    ilu.SetWatchdog(0.0)
+    P.fill_csr(ilu.GetMatrix(), frp, fci, fval)  # Factorize left the factors in F
+    assert ilu.Factorize() == -1
+    with pytest.raises(RuntimeError):
+        ilu.Substitute(b)
+    g = P.PreconditionedGmres()
+    g.SetIluPreconditioner(ilu)
+    ok, its = g.Solve(P.HostCsr.from_csr(a), b, np.zeros(a.n))
+    assert not ok
+    # the analysis waits too: with a zero limit Setup fails and leaves
+    # nothing to factorize, even with the limit raised again
+    assert ilu.SetupMatrix() == -1
+    ilu.SetWatchdog(10.0)
+    P.fill_csr(ilu.GetMatrix(), frp, fci, fval)
+    assert ilu.Factorize() == -1
+    with pytest.raises(RuntimeError):
+        ilu.Substitute(b)
+    assert ilu.SetupMatrix() == 0
+    assert ilu.Factorize() == 0
+    assert np.array_equal(host_factors(ilu, fci.size), ofac)
+    assert np.array_equal(ilu.Substitute(b), x_ok)
+    with pytest.raises(RuntimeError):
+        ilu.SetWatchdog(-1.0)
+
+
+def test_level_walk_refuses_oversized_descriptor_table():
+    """The level walk keeps one descriptor per (level, row slot): a chain
+    (one row per level) with 4 lanes per row (256 slots) would need 256
+    descriptors per row.  It is refused with an error instead of an
+    allocation that grows with levels x slots; 64 lanes (16 slots) fit."""
+    import scipy.sparse as sp
+
+    n = 1 << 17
+    m = sp.diags([-1.0, 2.5], [-1, 0], shape=(n, n), format="csr")
+    m.sort_indices()
+    rp, ci, va = (np.ascontiguousarray(v) for v in (m.indptr.astype(np.int32), m.indices.astype(np.int32), m.data))
+    ilu = factored(rp, ci, va)
+    ilu.SetSweepLanes(4, 4)
+    with pytest.raises(RuntimeError):
+        ilu.SetSweepMode(4, 4, 0)
+        ilu.Substitute(np.ones(n))
+    ilu.SetSweepLanes(64, 64)
+    ilu.SetSweepMode(4, 4, 0)
+    x = ilu.Substitute(np.ones(n))
+    assert np.all(np.isfinite(x))

@@ -91,6 +91,28 @@ def test_compensated_factorization_matches_oracle(oracle):
     assert np.array_equal(ilu.DeviceFactors(), plain)
 
 
+@pytest.mark.parametrize("lof", [1, 2])
+def test_compensated_ilu1_factorization_matches_oracle(oracle, lof):
+    """The per-row compensation on an ILU(k) pattern with fill (the learned
+    form runs compensated ILU(1)): the dropped updates are those falling
+    outside F's pattern, and the factors match the oracle's to 1e-12."""
+    a = gen.convdiff3d(9)
+    o = oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, lof)
+    frp, fci, fval = o[0], o[1], o[3]
+    omega = np.random.default_rng(lof).uniform(0.0, 1.0, a.n).astype(np.float32)
+    rp, ci, va, om = dev(a.row_ptr, a.col_idx, a.value, omega)
+    ilu = P.IluSolver(0)
+    assert ilu.SymbolicDevice(a.n, rp.data_ptr(), ci.data_ptr(), lof) == fci.size
+    ilu.SetDiagonalCompensation(om.data_ptr())
+    assert ilu.FactorizeDevice(va.data_ptr(), True) == 0
+    fac = ilu.DeviceFactors()
+    ref, _ = oracle.ilu_factorize_comp(frp, fci, fval, omega)
+    plain, _, _ = oracle.ilu_factorize(frp, fci, fval)
+    assert np.max(np.abs(fac - ref) / np.maximum(np.abs(ref), 1e-300)) <= 1e-12
+    assert not np.allclose(ref, plain)  # the fill dropped at this level changes the pivots
+    ilu.SetDiagonalCompensation(0)
+
+
 def test_learned_solve_fallback():
     a = grf(12, seed=7)
     b = gen.rhs_uniform(a.n, seed=7)
