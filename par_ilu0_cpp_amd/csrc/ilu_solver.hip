@@ -844,9 +844,13 @@ void sweep_pp_dispatch(int lanes, Ext* e, const double* b, double* out, double* 
     }
 }
 
-// Most (level, row slot) descriptors the level walk may allocate per
-// direction: 32 per row (a 16-byte descriptor each), at least 2^20.
-long lv_desc_cap(long n) { return std::max(32L * n, 1L << 20); }
+// Most (level, row slot) descriptors (16 bytes each) the level walk may
+// allocate per direction: 64 per row, or 2^26 (1 GiB) for small n.  A
+// chain-like schedule (levels close to n) with many row slots per level
+// exceeds it and is refused; the autotuner only tries shapes within half
+// of the per-row budget.
+long lv_desc_cap(long n) { return std::max(64L * n, 1L << 26); }
+long lv_desc_tune_cap(long n) { return 32L * n; }
 
 template <int G, typename T, bool UPPER>
 void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* reset, const int* gate) {
@@ -1268,7 +1272,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     if (e->n / deep <= kLvTryRows) {
         e->mode_l = e->mode_u = 4;
         for (int g : {4, 8, 16, 32, 64}) {
-            if (static_cast<long>(deep) * (pilu::kLvThreads / g) > pilu::lv_desc_cap(e->n)) continue;
+            if (static_cast<long>(deep) * (pilu::kLvThreads / g) > pilu::lv_desc_tune_cap(e->n)) continue;
             double tl, tu;
             pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
             record(4, Part{0, 0, 0, 0}, g, tl, tu);

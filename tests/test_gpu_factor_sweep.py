@@ -544,7 +544,7 @@ def test_watchdog_zero_refuses_work(oracle):
 
 def test_level_walk_refuses_oversized_descriptor_table():
     """The level walk keeps one descriptor per (level, row slot): a chain
-    (one row per level) with 4 lanes per row (256 slots) would need 256
+    (one row per level) with 1 lane per row (1024 slots) would need 1024
     descriptors per row.  It is refused with an error instead of an
     allocation that grows with levels x slots; 64 lanes (16 slots) fit."""
     import scipy.sparse as sp
@@ -554,7 +554,7 @@ def test_level_walk_refuses_oversized_descriptor_table():
     m.sort_indices()
     rp, ci, va = (np.ascontiguousarray(v) for v in (m.indptr.astype(np.int32), m.indices.astype(np.int32), m.data))
     ilu = factored(rp, ci, va)
-    ilu.SetSweepLanes(4, 4)
+    ilu.SetSweepLanes(1, 1)
     with pytest.raises(RuntimeError):
         ilu.SetSweepMode(4, 4, 0)
         ilu.Substitute(np.ones(n))
