@@ -54,6 +54,12 @@ public:
     bool FactorizeDevice(const double* d_values);
     // Apply the preconditioner to device vectors; b may alias x.
     void SubstituteDevice(const double* d_b, double* d_x);
+    // The same apply without allocating device memory: the intermediate
+    // vector and the sweeps' counters live in a static per-device scratch
+    // (n <= 4194304).  The sweep structures of the current configuration
+    // must exist (one SubstituteDevice call builds them); otherwise it
+    // throws.  Calls on one device share the scratch and must not overlap.
+    void SubstituteDeviceNoAlloc(const double* d_b, double* d_x);
     // 64 (default) or 32: precision in which the factors are stored for the
     // triangular sweeps.  Vectors and accumulation stay FP64.
     void SetFactorPrecision(int bits);

@@ -101,6 +101,20 @@ int IluSolverFactorizeDeviceEx(IluSolverHdl hdl, const double* d_values, int val
 #define IluSolverSubstituteDeviceEx LLJBASH_DECL(IluSolverSubstituteDeviceEx)
 int IluSolverSubstituteDeviceEx(IluSolverHdl hdl, const double* d_b, double* d_x);
 
+/* The apply of IluSolverSubstituteDeviceEx without any device allocation,
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  No counterpart in the reference.  The forward sweep's
+ * intermediate vector, the ticket counters and the level walk's progress
+ * words live in one static device array per GPU (32 MiB, held until the
+ * process ends), so n must not exceed 4194304 rows (-1 otherwise).  The
+ * sweep structures of the current sweep configuration must already exist:
+ * run IluSolverSubstituteDeviceEx once after Setup/Factorize or after a
+ * change of sweep shape (-1 otherwise).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (several host threads or
+ * streams) race.  Results are bit-identical to IluSolverSubstituteDeviceEx. */
+#define IluSolverSubstituteNoAllocEx LLJBASH_DECL(IluSolverSubstituteNoAllocEx)
+int IluSolverSubstituteNoAllocEx(IluSolverHdl hdl, const double* d_b, double* d_x);
+
 /* Copies of the device factors (nnz(F) doubles) and fill levels
  * (nnz(F) bytes, after SymbolicDeviceEx) to host memory. */
 #define IluSolverCopyFactorsEx LLJBASH_DECL(IluSolverCopyFactorsEx)

@@ -168,6 +168,12 @@ class IluSolver:
         if fn("IluSolverSubstituteDeviceEx")(self.hdl, C.c_void_p(d_b), C.c_void_p(d_x)) != 0:
             raise RuntimeError("IluSolverSubstituteDeviceEx failed")
 
+    def SubstituteNoAlloc(self, d_b: int, d_x: int) -> None:
+        """SubstituteDevice without device allocation (static per-device
+        scratch; the sweep structures must already exist)."""
+        if fn("IluSolverSubstituteNoAllocEx")(self.hdl, C.c_void_p(d_b), C.c_void_p(d_x)) != 0:
+            raise RuntimeError("IluSolverSubstituteNoAllocEx failed")
+
     def SetFactorPrecision(self, bits: int) -> None:
         if fn("IluSolverSetFactorPrecisionEx")(self.hdl, int(bits)) != 0:
             raise RuntimeError("bad precision")

@@ -96,6 +96,7 @@ SIGNATURES = [
     ("IluSolverSymbolicDeviceEx", _int, [_vp, P(CsrMatrix), _int]),
     ("IluSolverFactorizeDeviceEx", _int, [_vp, _vp, _int]),
     ("IluSolverSubstituteDeviceEx", _int, [_vp, _vp, _vp]),
+    ("IluSolverSubstituteNoAllocEx", _int, [_vp, _vp, _vp]),
     ("IluSolverCopyFactorsEx", _int, [_vp, P(_dbl)]),
     ("IluSolverCopyFillLevelsEx", _int, [_vp, P(C.c_byte)]),
     ("IluSolverNonzerosEx", _int, [_vp]),

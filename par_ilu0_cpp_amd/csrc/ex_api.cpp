@@ -102,6 +102,10 @@ int IluSolverSubstituteDeviceEx(IluSolverHdl hdl, const double* d_b, double* d_x
     return guarded("SubstituteDevice", [&] { S(hdl)->SubstituteDevice(d_b, d_x); return 0; });
 }
 
+int IluSolverSubstituteNoAllocEx(IluSolverHdl hdl, const double* d_b, double* d_x) {
+    return guarded("SubstituteNoAlloc", [&] { S(hdl)->SubstituteDeviceNoAlloc(d_b, d_x); return 0; });
+}
+
 int IluSolverCopyFactorsEx(IluSolverHdl hdl, double* out) {
     return guarded("CopyFactors", [&] {
         const int nnz = S(hdl)->Nonzeros();

@@ -120,6 +120,11 @@ struct lljbash::IluSolver::Ext {
     int part_version = 0;
     long factor_version = 0;
     pilu::DevBuf<int> bperm_l, bperm_u, blev_l, blev_u, loff_l, loff_u;
+    // Set while a no-allocation apply runs (SubstituteDeviceNoAlloc): a
+    // sweep structure that would have to be built then raises instead, and
+    // the level walk keeps its progress words in lvprog_scratch.
+    bool no_alloc = false;
+    unsigned long long* lvprog_scratch = nullptr;
     bool analysed = false;  // pattern checked and schedules built (SetupMatrix / SymbolicDevice completed)
     bool factored = false;
     bool y_armed = false;

@@ -62,6 +62,17 @@ void check_sweep_errors(lljbash::IluSolver::Ext* e) {
     }
 }
 
+double* device_scratch();
+
+// Raised when a sweep structure is missing during a no-allocation apply
+// (building it would allocate device memory).
+void refuse_if_no_alloc(const lljbash::IluSolver::Ext* e, const char* what) {
+    if (e->no_alloc)
+        throw std::runtime_error(std::string("no-allocation apply: ") + what +
+                                 " not built for the current sweep configuration (run IluSolverSubstituteDeviceEx "
+                                 "once first)");
+}
+
 namespace {
 
 // f(begin, end) over [0, n) in contiguous chunks on up to 16 host threads.
@@ -498,6 +509,7 @@ void analyse(Ext* e, const int* host_rp) {
     e->max_lo = max_lo;
     e->max_up = max_up;
 
+    (void)device_scratch();  // the no-allocation apply's scratch on this device
     e->nlev_l = build_schedule(e, false, e->perm_l, e->lvoff_l);
     e->nlev_u = build_schedule(e, true, e->perm_u, e->lvoff_u);
     ++e->pattern_version;
@@ -571,6 +583,7 @@ __global__ void k_pack_fill(int n, const int* __restrict__ perm, const int* __re
 }
 
 void build_pack(Ext* e, bool upper, SweepPack& pk) {
+    refuse_if_no_alloc(e, "packed factors");
     const int n = e->n, nblk = e->nblk;
     const DevBuf<int>& perm = upper ? e->bperm_u : e->bperm_l;
     const DevBuf<int>& blev = upper ? e->blev_u : e->blev_l;
@@ -657,6 +670,7 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
 // kPpThreads / G rows, each block's list padded to a multiple of kPpUnroll.
 void build_passes(Ext* e, SweepPack& pk, int G) {
     if (pk.pass_g == G) return;
+    refuse_if_no_alloc(e, "pass list");
     const int per = kPpThreads / G;
     std::vector<PassDesc> desc;
     std::vector<int> dptr(static_cast<size_t>(e->nblk) + 1, 0);
@@ -702,6 +716,7 @@ void build_passes(Ext* e, SweepPack& pk, int G) {
 // the U diagonal even when a direction has no off-diagonal entry at all.
 void gather_pack(Ext* e, SweepPack& pk, bool upper) {
     if (pk.version != e->part_version) return;
+    refuse_if_no_alloc(e, "packed factor values");
     hipStream_t s = e->stream;
     const int grid = 2048;
     if (e->precision == 32) {
@@ -868,11 +883,13 @@ void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* re
         throw std::runtime_error("level-walk sweep: levels x row slots exceed the descriptor budget "
                                  "(use fewer lanes per row or another sweep mode)");
     if (e->lvprog.size() < 3) {
+        refuse_if_no_alloc(e, "level-walk progress words");
         e->lvprog.reserve(3);
         PILU_HIP(hipMemset(e->lvprog.get(), 0, 3 * sizeof(unsigned long long)));
     }
     int& cached = UPPER ? e->lvcached_u : e->lvcached_l;
     if (key[0] != e->pattern_version || key[1] != slots) {
+        refuse_if_no_alloc(e, "level-walk descriptors");
         int* nbad = reinterpret_cast<int*>(e->lvprog.get() + 2);
         e->lvbad.reserve(static_cast<size_t>(nlev));
         PILU_HIP(hipMemsetAsync(nbad, 0, sizeof(int), e->stream));
@@ -891,12 +908,13 @@ void sweep_lv_one(Ext* e, const T* val, const double* b, double* out, double* re
     // warm its L2; the others return at once
     const int grid = 1 + 8 * kLvHelpers;
     const unsigned epoch = ++e->lv_epoch;
+    unsigned long long* prog = e->lvprog_scratch ? e->lvprog_scratch : e->lvprog.get();
     auto kern = k_sweep_lv<G, T, UPPER>;
     PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(kLvLds)));
     kern<<<grid, kLvThreads, kLvLds, e->stream>>>(nlev, off.get(), desc.get(), perm.get(), e->rp.get(), e->ci.get(),
-                                                 e->dg.get(), val, e->nnz, b, out, reset, gate, e->lvprog.get(), epoch,
-                                                 reinterpret_cast<int*>(e->lvprog.get() + 1),
+                                                 e->dg.get(), val, e->nnz, b, out, reset, gate, prog, epoch,
+                                                 reinterpret_cast<int*>(prog + 1),
                                                  watchdog_ticks(e->watchdog_s), cached > 0 ? 1 : 0);
 }
 
@@ -1025,6 +1043,72 @@ void ilu_apply(Ext* e, const double* d_b, double* d_x, const int* gate) {
     } else {
         sweep_dispatch<double, false>(gl, e, e->fac.get(), d_b, e->y.get(), d_x, gate, ctr);
         sweep_dispatch<double, true>(gu, e, e->fac.get(), e->y.get(), d_x, e->y.get(), gate, ctr + 2);
+    }
+    PILU_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------
+// Scratch of the no-allocation apply: the forward sweep's output y
+// (kScratchRows doubles), then 8 words for the level walk's progress (words
+// 0-2) and 16 ticket counters (as ints, words 8-15).  One copy per device,
+// held until the process ends; every no-allocation apply on a device uses
+// it, so such calls must not overlap on one device.
+// ---------------------------------------------------------------------
+constexpr long kScratchRows = 1L << 22;
+constexpr long kScratchWords = kScratchRows + 16;
+static __device__ __align__(256) double scratch[kScratchWords];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (the host-side symbol is only a
+// stand-in and must never be passed to a kernel).
+double* device_scratch() {
+    int dev = 0;
+    PILU_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) throw std::runtime_error("no-allocation apply: device index too large");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        PILU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("no-allocation apply: scratch symbol has no device address");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// x = U^-1 L^-1 b like ilu_apply, with y, the ticket counters and the level
+// walk's progress words in the device scratch instead of the solver's own
+// buffers; allocates nothing.  Every sweep structure the configured sweep
+// needs (packs, pass lists, level-walk descriptors) must already exist.
+void ilu_apply_no_alloc(Ext* e, const double* d_b, double* d_x) {
+    need_setup(e);
+    if (!e->factored) throw std::runtime_error("IluSolver: Factorize has not been called");
+    if (e->n > kScratchRows)
+        throw std::runtime_error("no-allocation apply: n exceeds the scratch capacity of 4194304 rows");
+    double* base = device_scratch();
+    double* y = base;
+    unsigned long long* prog = reinterpret_cast<unsigned long long*>(base + kScratchRows);
+    int* ctr = reinterpret_cast<int*>(base + kScratchRows + 8);
+    struct Guard {
+        Ext* e;
+        Guard(Ext* x, unsigned long long* p) : e(x) { e->no_alloc = true; e->lvprog_scratch = p; }
+        ~Guard() { e->no_alloc = false; e->lvprog_scratch = nullptr; }
+    } guard(e, prog);
+    hipStream_t s = e->stream;
+    // y starts all-pending; progress words and ticket counters start at
+    // zero (fill kernels, so the sequence can also be captured in a graph)
+    k_fill_u64<<<grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(y), kPending);
+    k_fill_u64<<<1, 64, 0, s>>>(16, reinterpret_cast<uint64_t*>(prog), 0);
+    const int gl = effective_lanes(e, false), gu = effective_lanes(e, true);
+    if (e->precision == 32) {
+        sweep_dispatch<float, false>(gl, e, e->fac32.get(), d_b, y, d_x, nullptr, ctr);
+        sweep_dispatch<float, true>(gu, e, e->fac32.get(), y, d_x, y, nullptr, ctr + 2);
+    } else {
+        sweep_dispatch<double, false>(gl, e, e->fac.get(), d_b, y, d_x, nullptr, ctr);
+        sweep_dispatch<double, true>(gu, e, e->fac.get(), y, d_x, y, nullptr, ctr + 2);
     }
     PILU_HIP(hipGetLastError());
 }
@@ -1163,6 +1247,10 @@ void IluSolver::Substitute(const double* b, double* x) {
 
 void IluSolver::SubstituteDevice(const double* d_b, double* d_x) {
     pilu::ilu_apply(pilu::need_setup(ext_), d_b, d_x, nullptr);
+}
+
+void IluSolver::SubstituteDeviceNoAlloc(const double* d_b, double* d_x) {
+    pilu::ilu_apply_no_alloc(pilu::need_setup(ext_), d_b, d_x);
 }
 
 void IluSolver::SetWatchdog(double seconds) {
