@@ -1,53 +1,78 @@
 """Writes the symbolic-ILU(k) golden fixtures (tests/golden/iluk_*.npz).
 
-Each fixture holds an input pattern A (row_ptr, col_idx, value) and the
-output of the reference's own src/iluk.cpp for it -- compiled from the
-reference sources into oracle/_ref/libref_iluk.so by `make -C oracle ref`
--- at one level of fill: the pattern of F, the nzmap, and F's values at the
-level-0 slots (the reference leaves fill slots uninitialised, so only the
-level-0 slots are stored, as a mask plus values).
+Each fixture holds an input pattern A (row_ptr, col_idx, value) and, for
+several levels of fill, the pattern of F, the nzmap (index in F of every
+entry of A) and F's values at the level-0 slots.  They are computed by
+`first_found_rows` below: a restatement of the reference's level-of-fill
+rule (src/iluk.cpp:14-28) written independently of the C oracle -- a dict
+per row plus a sorted key list, re-searched after every merge so that
+columns inserted during the pivot walk are visited, as iterating a
+std::map does, and a level recorded the first time a column is reached
+(try_emplace).  The fixtures fix the oracle's and the GPU pass's outputs on
+these inputs; the rule itself is pinned by the hand-derived known-answer
+tests in tests/test_oracle_symbolic.py.
 
-Run from the repository root where /root/reference is present:
-    make -C oracle ref && python tests/golden/make_golden.py
+Run from the repository root:  python tests/golden/make_golden.py
 """
 from __future__ import annotations
 
-import ctypes as C
+import bisect
 import sys
 from pathlib import Path
 
 import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
-sys.path.insert(0, str(ROOT))
-
-from par_ilu0_cpp_amd import HostCsr  # noqa: E402
-from par_ilu0_cpp_amd import generators as gen  # noqa: E402
-from par_ilu0_cpp_amd._lib import CsrMatrix  # noqa: E402
-from tests.conftest import dominant_values, random_pattern  # noqa: E402
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
 
 OUT = Path(__file__).resolve().parent
 
 
-def ref_lib():
-    lib = C.CDLL(str(ROOT / "oracle" / "_ref" / "libref_iluk.so"))
-    f = lib.LLJBASH_IlukSymbolic
-    f.argtypes = [C.POINTER(CsrMatrix), C.c_int, C.POINTER(CsrMatrix), C.POINTER(C.c_int)]
-    f.restype = C.c_int
-    return f
+def first_found_rows(rp, ci, lof):
+    """Rows of F as {column: level} dicts under the first-found rule."""
+    n = len(rp) - 1
+    rows = []
+    for i in range(n):
+        lev = {int(c): 0 for c in ci[rp[i]:rp[i + 1]]}
+        keys = sorted(lev)
+        pos = 0
+        while pos < len(keys):
+            k = keys[pos]
+            if k >= i:
+                break
+            if lev[k] < lof and k in rows[k]:
+                for j in sorted(rows[k]):
+                    if j < k:
+                        continue
+                    level = lev[k] + rows[k][j] + 1
+                    if level <= lof and j not in lev:
+                        lev[j] = level
+                        bisect.insort(keys, j)
+            pos = bisect.bisect_right(keys, k)
+        rows.append(lev)
+    return rows
 
 
-def run_ref(fn, rp, ci, va, lof):
-    a = HostCsr(rp, ci, va)
-    f = HostCsr()
-    nz = np.zeros(max(ci.size, 1), dtype=np.int32)
-    k = fn(C.byref(a.mat), int(lof), C.byref(f.mat), nz.ctypes.data_as(C.POINTER(C.c_int)))
-    frp, fci, fva = f.arrays()
-    assert k == fci.size
-    return frp, fci, fva, nz[: ci.size]
+def symbolic(rp, ci, va, lof):
+    """(row_ptr, col_idx, nzmap, level-0 values) of F from first_found_rows."""
+    rows = first_found_rows(rp, ci, lof)
+    frp = np.zeros(len(rows) + 1, dtype=np.int32)
+    frp[1:] = np.cumsum([len(r) for r in rows])
+    fci = np.array([c for r in rows for c in sorted(r)], dtype=np.int32)
+    nz = np.empty(len(ci), dtype=np.int32)
+    for i in range(len(rows)):
+        cols = fci[frp[i]:frp[i + 1]]
+        for p in range(rp[i], rp[i + 1]):
+            nz[p] = frp[i] + int(np.searchsorted(cols, ci[p]))
+    order = np.argsort(nz)
+    return frp, fci, nz, np.asarray(va, dtype=np.float64)[order]
 
 
 def cases():
+    from par_ilu0_cpp_amd import generators as gen
+    from tests.conftest import dominant_values, random_pattern
+
     # hand-derived first-found case (n=7): see tests/test_oracle_symbolic.py
     rows = {0: [0, 5], 1: [1], 2: [0, 2], 3: [3, 5], 4: [2, 3, 4], 5: [5], 6: [4, 6]}
     rp = np.zeros(8, dtype=np.int32)
@@ -66,17 +91,14 @@ def cases():
 
 
 def main():
-    fn = ref_lib()
     for name, rp, ci, va, lofs in cases():
         data = {"row_ptr": rp, "col_idx": ci, "value": va}
         for lof in lofs:
-            frp, fci, fva, nz = run_ref(fn, rp, ci, va, lof)
-            level0 = np.zeros(fci.size, dtype=bool)
-            level0[nz] = True
+            frp, fci, nz, level0 = symbolic(rp, ci, va, lof)
             data[f"f{lof}_row_ptr"] = frp
             data[f"f{lof}_col_idx"] = fci
             data[f"f{lof}_nzmap"] = nz
-            data[f"f{lof}_level0_values"] = fva[level0]
+            data[f"f{lof}_level0_values"] = level0
         data["lofs"] = np.array(lofs, dtype=np.int32)
         np.savez_compressed(OUT / f"iluk_{name}.npz", **data)
         print(name, {lof: int(data[f'f{lof}_row_ptr'][-1]) for lof in lofs})

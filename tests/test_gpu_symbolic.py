@@ -21,7 +21,7 @@ def gpu_symbolic(rp, ci, va, lof):
 
 
 @pytest.mark.parametrize("path", sorted(GOLDEN.glob("iluk_*.npz")), ids=lambda p: p.stem)
-def test_matches_reference_golden(path):
+def test_matches_golden(path):
     d = np.load(path)
     for lof in d["lofs"]:
         frp, fci, fva, flev, nz = gpu_symbolic(d["row_ptr"], d["col_idx"], d["value"], int(lof))
@@ -97,3 +97,21 @@ def test_full_size_128_cube_ilu1_closed_form():
     assert ilu.Levels() == (6 * s - 5, 6 * s - 5)
     lev = ilu.FillLevels()
     assert lev.size == nnz and int(np.count_nonzero(lev == 0)) == a.nnz and int(lev.max()) == 1
+
+
+@pytest.mark.parametrize("lof", [1, 2, 3])
+def test_kat9_first_found_levels(lof):
+    """The hand-derived n = 9 case (tests/test_oracle_symbolic.py) on the GPU:
+    the first level found within lof is kept (row 3's (3,6) at 2, not 1),
+    so (4,6), (5,6) appear only at lof 3 and (7,6) never; levels and
+    pattern equal the independent restatement's."""
+    from tests.golden.make_golden import first_found_rows
+    from tests.test_oracle_symbolic import kat9
+
+    rp, ci = kat9()
+    frp, fci, fva, flev, nz = gpu_symbolic(rp, ci, np.arange(1.0, ci.size + 1), lof)
+    want = first_found_rows(rp, ci, lof)
+    assert frp[-1] == {1: 20, 2: 20, 3: 22}[lof]
+    for i, r in enumerate(want):
+        got = {int(c): int(l) for c, l in zip(fci[frp[i]:frp[i + 1]], flev[frp[i]:frp[i + 1]])}
+        assert got == r, i

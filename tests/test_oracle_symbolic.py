@@ -2,19 +2,23 @@
 src/iluk.cpp:5-54 of the reference).
 
 Checks, none of which re-types the oracle's own loop:
-  * a hand-derived known-answer test of the first-found level rule;
-  * golden fixtures written by the reference's own src/iluk.cpp
-    (tests/golden/make_golden.py), and a live comparison with that build
-    (oracle/_ref) when it is present;
+  * hand-derived known-answer tests of the first-found level rule (n = 7 at
+    lof 2; n = 9 at lof 1, 2 and 3, where the first-found and the
+    minimum-level rules differ in a level and in two entries), so that a
+    pivot-order or level mistake in the oracle fails a check derived by
+    hand;
+  * golden fixtures written by an independent Python restatement of the
+    rule (tests/golden/make_golden.py: a dict per row and a sorted key list
+    re-searched after every merge, the way iterating an ordered map visits
+    keys inserted during the walk), and a live comparison with that
+    restatement on random patterns;
   * closed-form entry counts of stencil patterns;
-  * ILU(0) = pattern of A, ILU(1) = textbook ILU(1), monotone growth in k;
-  * a dictionary-and-bisect restatement that iterates the row the way an
-    ordered map does (keys inserted during the walk are visited).
+  * ILU(0) = pattern of A, ILU(1) = textbook ILU(1), monotone growth in k.
+The reference's own iluk.cpp cannot be built here (it needs the
+reference's csr_matrix.cpp, which needs MKL and SuiteSparse).
 """
 from __future__ import annotations
 
-import bisect
-import ctypes as C
 from pathlib import Path
 
 import numpy as np
@@ -22,6 +26,7 @@ import pytest
 
 from par_ilu0_cpp_amd import generators as gen
 from tests.conftest import dominant_values, random_pattern
+from tests.golden.make_golden import first_found_rows
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 ROOT = Path(__file__).resolve().parent.parent
@@ -81,31 +86,37 @@ def test_kat_differs_from_minimum_level_rule():
     assert minimum_level_iluk(rp, ci, 2) == 16
 
 
-def ordered_map_restatement(rp, ci, lof):
-    """Row i as dict + sorted key list; the pivot walk looks up the next key
-    above the current one after every merge, so keys inserted during the
-    walk are visited, as std::map iteration does."""
-    n = rp.size - 1
-    rows = []
-    for i in range(n):
-        lev = {int(c): 0 for c in ci[rp[i]:rp[i + 1]]}
-        keys = sorted(lev)
-        pos = 0
-        while pos < len(keys):
-            k = keys[pos]
-            if k >= i:
-                break
-            if lev[k] < lof and k in rows[k]:
-                for j in sorted(rows[k]):
-                    if j < k:
-                        continue
-                    L = lev[k] + rows[k][j] + 1
-                    if L <= lof and j not in lev:
-                        lev[j] = L
-                        bisect.insort(keys, j)
-            pos = bisect.bisect_right(keys, k)
-        rows.append(lev)
-    return rows
+def kat9():
+    rows = {0: [0, 6], 1: [0, 1], 2: [2, 6], 3: [1, 2, 3], 4: [3, 4], 5: [3, 5], 6: [6, 8], 7: [4, 7], 8: [8]}
+    rp = np.zeros(10, dtype=np.int32)
+    rp[1:] = np.cumsum([len(rows[i]) for i in range(9)])
+    ci = np.array([c for i in range(9) for c in rows[i]], dtype=np.int32)
+    return rp, ci
+
+
+@pytest.mark.parametrize("lof,nnz,fill", [
+    # Row 1 gets (1,6)@1 through pivot 0.  Row 3 meets column 6 first
+    # through pivot 1 (0 + 1 + 1 = 2) and then through pivot 2 (0 + 0 + 1 =
+    # 1); the first level found within lof is kept.  Rows 4 and 5 reach 6
+    # only through pivot 3 at lev(3,6) + 1; row 7 only through pivot 4 at
+    # lev(4,6) + 1.
+    (1, 20, {1: {6: 1}, 3: {6: 1}}),                    # via pivot 1: 2 > 1, so pivot 2's level 1
+    (2, 20, {1: {6: 1}, 3: {6: 2}}),                    # (4,6), (5,6) would be 3 > 2
+    (3, 22, {1: {6: 1}, 3: {6: 2}, 4: {6: 3}, 5: {6: 3}}),  # (7,6) would be 4 > 3
+])
+def test_kat9_first_found_levels(oracle, lof, nnz, fill):
+    """Hand-derived first-found patterns and levels (every entry of A at
+    level 0, plus `fill`); the minimum-level rule gives (3,6)@1 and two more
+    entries at lof 2 ((4,6)@2, (5,6)@2), and (7,6)@3 on top at lof 3."""
+    rp, ci = kat9()
+    frp, fci, flev, fval, nz = oracle.iluk_symbolic(rp, ci, np.arange(1.0, ci.size + 1), lof)
+    assert frp[-1] == nnz
+    for i in range(9):
+        want = {int(c): 0 for c in ci[rp[i]:rp[i + 1]]}
+        want.update(fill.get(i, {}))
+        assert row_dict(frp, fci, flev, i) == want, i
+    assert np.array_equal(fval[nz], np.arange(1.0, ci.size + 1))
+    assert minimum_level_iluk(rp, ci, lof) == {1: 20, 2: 22, 3: 23}[lof]
 
 
 @pytest.mark.parametrize("lof", [0, 1, 2, 3])
@@ -113,13 +124,13 @@ def ordered_map_restatement(rp, ci, lof):
 def test_matches_ordered_map_restatement(oracle, lof, seed):
     rp, ci = random_pattern(120, 3 + seed, 15, seed=seed)
     frp, fci, flev, _, _ = oracle.iluk_symbolic(rp, ci, dominant_values(rp, ci, seed), lof)
-    rows = ordered_map_restatement(rp, ci, lof)
+    rows = first_found_rows(rp, ci, lof)
     for i, r in enumerate(rows):
         assert row_dict(frp, fci, flev, i) == r, i
 
 
 @pytest.mark.parametrize("path", sorted(GOLDEN.glob("iluk_*.npz")), ids=lambda p: p.stem)
-def test_matches_reference_golden(oracle, path):
+def test_matches_golden(oracle, path):
     d = np.load(path)
     for lof in d["lofs"]:
         frp, fci, flev, fval, nz = oracle.iluk_symbolic(d["row_ptr"], d["col_idx"], d["value"], int(lof))
@@ -128,37 +139,6 @@ def test_matches_reference_golden(oracle, path):
         assert np.array_equal(nz, d[f"f{lof}_nzmap"]), lof
         assert np.array_equal(fval[nz], d[f"f{lof}_level0_values"]), lof
         assert np.all(flev[nz] == 0) and np.count_nonzero(flev == 0) == nz.size
-
-
-def _ref_fn():
-    path = ROOT / "oracle" / "_ref" / "libref_iluk.so"
-    if not path.exists():
-        pytest.skip("oracle/_ref not built (needs the reference sources)")
-    from par_ilu0_cpp_amd._lib import CsrMatrix
-
-    lib = C.CDLL(str(path))
-    fn = lib.LLJBASH_IlukSymbolic
-    fn.argtypes = [C.POINTER(CsrMatrix), C.c_int, C.POINTER(CsrMatrix), C.POINTER(C.c_int)]
-    fn.restype = C.c_int
-    return fn
-
-
-@pytest.mark.parametrize("lof", [0, 1, 2, 3, 5])
-@pytest.mark.parametrize("seed", [11, 12])
-def test_matches_reference_build_live(oracle, lof, seed):
-    from par_ilu0_cpp_amd import HostCsr
-
-    fn = _ref_fn()
-    rp, ci = random_pattern(700, 5, 50, seed=seed)
-    va = dominant_values(rp, ci, seed)
-    a, f = HostCsr(rp, ci, va), HostCsr()
-    nz_ref = np.zeros(ci.size, dtype=np.int32)
-    k = fn(C.byref(a.mat), lof, C.byref(f.mat), nz_ref.ctypes.data_as(C.POINTER(C.c_int)))
-    rrp, rci, rva = f.arrays()
-    frp, fci, flev, fval, nz = oracle.iluk_symbolic(rp, ci, va, lof)
-    assert k == frp[-1]
-    assert np.array_equal(rrp, frp) and np.array_equal(rci, fci) and np.array_equal(nz_ref, nz)
-    assert np.array_equal(rva[nz], fval[nz])
 
 
 def test_ilu0_is_pattern_of_a(oracle):
