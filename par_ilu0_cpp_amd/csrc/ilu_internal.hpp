@@ -58,6 +58,26 @@ struct SweepPack {
     int max_np = 0;                 // most PassDesc records of one block
     int pass_g = 0;                 // lanes per row the pass list was built for
 };
+// Record streams of the wave-specialised sweep (mode 5) for one direction,
+// built for a partition and a lanes-per-row count.
+struct WsPack {
+    DevBuf<uint4> stream;           // records of every block, blocks in processing order
+    DevBuf<int> vdst, vsrc;         // value slot (in T units of the stream) <- factor entry (-1: 0)
+    DevBuf<int2> imports;           // per import: face slot of the imported row, first record using it
+    DevBuf<char> blocks;            // WsBlock per processed block
+    DevBuf<double> face;            // face buffer (value-flags of the rows other blocks read)
+    long nvals = 0;
+    long nstream = 0;               // 16-byte units
+    int nface = 0;
+    int max_imp = 0;
+    int value_bytes = 0;            // 8 or 4: precision the stream was laid out for
+    int version = -1;               // partition the stream was built for
+    int g = 0;                      // lanes per row it was built for
+    long factors = -1;              // factorization its values were gathered from
+    bool face_armed = false;        // every face slot holds kPending
+    std::vector<int> face_rng;      // per block id: first and end face slot (independent of g)
+    int rng_version = -1;           // partition face_rng belongs to
+};
 }  // namespace pilu
 
 struct lljbash::IluSolver::Ext {
@@ -104,9 +124,11 @@ struct lljbash::IluSolver::Ext {
     // 1 = block-local over contiguous blocks of blk_rows rows (k_sweep_blk),
     // 2 = packed block-local over the current partition (k_sweep_pp, level
     // barriers), 3 = the same with LDS value-flags, 4 = one workgroup walking
-    // the level schedule (k_sweep_lv; no partition).
+    // the level schedule (k_sweep_lv; no partition), 5 = wave-specialised
+    // block sweep over record streams (k_sweep_ws).
     int mode_l = 0, mode_u = 0;
     pilu::SweepPack pack_l, pack_u;
+    pilu::WsPack ws_l, ws_u;
     // Block partition of the rows: contiguous blocks of blk_rows rows, or
     // tiles of the grid (tile[] != 0, grid[] declared by SetGridShape).
     int blk_rows = 0, nblk = 0, max_block_rows = 0;

@@ -17,6 +17,7 @@
 
 #include "ilu_internal.hpp"
 #include "ilu_kernels.hpp"
+#include "ilu_sweep_ws.hpp"
 #include "iluk_device.hpp"
 
 namespace pilu {
@@ -746,6 +747,379 @@ void ensure_packs(Ext* e) {
     }
 }
 
+// ---------------------------------------------------------------------
+// Record streams of the wave-specialised sweep (mode 5), both directions
+// for the current partition and G lanes per row.  Per direction:
+//   * the block schedule (rows by processing position of their block,
+//     local level, row) comes from the device;
+//   * a block's imports are its distinct out-of-block columns, ordered by
+//     the first record that reads them; a row some block imports is a face
+//     row, and the face rows of each block get consecutive face slots in
+//     schedule order (so a block publishes and re-arms contiguous ranges);
+//   * each local level is cut into records of at most 64 / G rows; columns
+//     become LDS slots (in-block: schedule position within the block;
+//     imports: max_rows + import index; padding: the zero slot); values and
+//     diagonal are gathered into the stream after every factorization
+//     through (vdst, vsrc).
+// ---------------------------------------------------------------------
+struct WsDir {
+    std::vector<int> perm, blev, loff, kinv, blk_at, face_of;
+    std::vector<std::vector<int2>> imp;  // per position: (column, first record)
+    std::vector<int> face_lo;            // per position: first face slot (nblk + 1)
+};
+
+void ws_schedule(Ext* e, bool upper, WsDir& d, int P) {
+    const int n = e->n, nblk = e->nblk;
+    const DevBuf<int>& perm = upper ? e->bperm_u : e->bperm_l;
+    const DevBuf<int>& blev = upper ? e->blev_u : e->blev_l;
+    const DevBuf<int>& loff = upper ? e->loff_u : e->loff_l;
+    d.perm.resize(n);
+    d.blev.resize(static_cast<size_t>(nblk) + 1);
+    PILU_HIP(hipMemcpy(d.perm.data(), perm.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
+    PILU_HIP(hipMemcpy(d.blev.data(), blev.get(), sizeof(int) * d.blev.size(), hipMemcpyDeviceToHost));
+    d.loff.resize(static_cast<size_t>(d.blev[nblk]) + 1);
+    PILU_HIP(hipMemcpy(d.loff.data(), loff.get(), sizeof(int) * d.loff.size(), hipMemcpyDeviceToHost));
+    d.kinv.assign(n, 0);
+    for (int k = 0; k < n; ++k) d.kinv[d.perm[k]] = k;
+    const std::vector<int>& rp = e->h_rp;
+    const std::vector<int>& dg = e->h_dg;
+    const std::vector<int>& ci = e->h_ci;
+    const std::vector<int>& bo = e->h_block_of;
+    d.blk_at.assign(nblk, -1);
+    d.imp.assign(nblk, {});
+    parallel_for(nblk, [&](long lo, long hi) {
+        for (long p = lo; p < hi; ++p) {
+            const int g0 = d.blev[p], g1 = d.blev[p + 1];
+            if (g0 == g1) continue;
+            const int blk = bo[d.perm[d.loff[g0]]];
+            d.blk_at[p] = blk;
+            std::vector<int2> v;
+            int rec = 0;
+            for (int g = g0; g < g1; ++g)
+                for (int c0 = d.loff[g]; c0 < d.loff[g + 1]; c0 += P, ++rec)
+                    for (int k = c0; k < std::min(c0 + P, d.loff[g + 1]); ++k) {
+                        const int row = d.perm[k];
+                        const int e0 = upper ? dg[row] + 1 : rp[row], e1 = upper ? rp[row + 1] : dg[row];
+                        for (int q = e0; q < e1; ++q)
+                            if (bo[ci[q]] != blk) v.push_back(make_int2(ci[q], rec));
+                    }
+            std::sort(v.begin(), v.end(), [](int2 a, int2 b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+            size_t m = 0;
+            for (size_t i = 0; i < v.size(); ++i)
+                if (m == 0 || v[i].x != v[m - 1].x) v[m++] = v[i];
+            v.resize(m);
+            std::sort(v.begin(), v.end(), [](int2 a, int2 b) { return a.y != b.y ? a.y < b.y : a.x < b.x; });
+            d.imp[p] = std::move(v);
+        }
+    });
+    std::vector<char> is_face(n, 0);
+    for (const auto& v : d.imp)
+        for (const int2& x : v) is_face[x.x] = 1;
+    d.face_of.assign(n, -1);
+    d.face_lo.assign(static_cast<size_t>(nblk) + 1, 0);
+    int nf = 0;
+    for (int p = 0; p < nblk; ++p) {
+        d.face_lo[p] = nf;
+        const int k0 = d.loff[d.blev[p]], k1 = d.loff[d.blev[p + 1]];
+        for (int k = k0; k < k1; ++k)
+            if (is_face[d.perm[k]]) d.face_of[d.perm[k]] = nf++;
+    }
+    d.face_lo[nblk] = nf;
+}
+
+inline int a16(int b) { return (b + 15) & ~15; }
+
+// Lays out and uploads one direction's stream; other_face[2 b], [2 b + 1]:
+// the other direction's face slots of block b, which this direction
+// re-arms.
+void ws_stream(Ext* e, bool upper, const WsDir& d, const std::vector<int>& other_face, int G, WsPack& wp) {
+    const int n = e->n, nblk = e->nblk, P = 64 / G;
+    const int sv = e->precision == 32 ? 4 : 8;
+    const std::vector<int>& rp = e->h_rp;
+    const std::vector<int>& dg = e->h_dg;
+    const std::vector<int>& ci = e->h_ci;
+    const std::vector<int>& bo = e->h_block_of;
+    const int max_rows = e->max_block_rows;
+    int max_imp = 0;
+    for (const auto& v : d.imp) max_imp = std::max(max_imp, static_cast<int>(v.size()));
+    const int zero = max_rows + max_imp;
+    auto entries = [&](int row) { return upper ? rp[row + 1] - dg[row] - 1 : dg[row] - rp[row]; };
+    // pass 1: bytes and value slots per position
+    std::vector<long> bytes(nblk, 0), nval(nblk, 0);
+    std::vector<int> maxrec(nblk, 0);
+    parallel_for(nblk, [&](long lo, long hi) {
+        for (long p = lo; p < hi; ++p) {
+            long off = 0, nv = 0;
+            int big = 0;
+            for (int g = d.blev[p]; g < d.blev[p + 1]; ++g)
+                for (int c0 = d.loff[g]; c0 < d.loff[g + 1]; c0 += P) {
+                    const int cnt = std::min(P, d.loff[g + 1] - c0);
+                    int w = 0;
+                    for (int k = c0; k < c0 + cnt; ++k) w = std::max(w, entries(d.perm[k]));
+                    w = (w + G - 1) / G * G;
+                    const int rb = 16 + a16(cnt * 8) + a16(cnt * w * 4) + a16(cnt * w * sv) + (upper ? a16(cnt * sv) : 0);
+                    big = std::max(big, rb);
+                    if ((off % kWsRing) + rb > kWsRing) off += kWsRing - off % kWsRing;
+                    off += rb;
+                    nv += static_cast<long>(cnt) * w + (upper ? cnt : 0);
+                }
+            bytes[p] = (off + kWsChunk - 1) / kWsChunk * kWsChunk;
+            nval[p] = nv;
+            maxrec[p] = big;
+        }
+    });
+    if (*std::max_element(maxrec.begin(), maxrec.end()) > kWsRing / 2)
+        throw std::runtime_error("wave-specialised sweep: a record exceeds half the LDS ring (rows too long)");
+    std::vector<long> s0(static_cast<size_t>(nblk) + 1, 0), v0(static_cast<size_t>(nblk) + 1, 0),
+        i0(static_cast<size_t>(nblk) + 1, 0);
+    for (int p = 0; p < nblk; ++p) {
+        s0[p + 1] = s0[p] + bytes[p] / 16;
+        v0[p + 1] = v0[p] + nval[p];
+        i0[p + 1] = i0[p] + static_cast<long>(d.imp[p].size());
+    }
+    if (s0[nblk] >= (1L << 31) / static_cast<long>(16 / sv) || v0[nblk] >= (1L << 31) || i0[nblk] >= (1L << 31))
+        throw std::runtime_error("wave-specialised sweep: record stream too large");
+    std::vector<uint32_t> words(static_cast<size_t>(std::max<long>(s0[nblk], 1)) * 4, 0);
+    std::vector<int> vdst(static_cast<size_t>(std::max<long>(v0[nblk], 1))), vsrc(vdst.size(), -1);
+    std::vector<int2> imports(static_cast<size_t>(std::max<long>(i0[nblk], 1)));
+    std::vector<WsBlock> blocks(nblk);
+    // pass 2: fill
+    parallel_for(nblk, [&](long lo, long hi) {
+        std::vector<std::pair<int, int>> imap;
+        for (long p = lo; p < hi; ++p) {
+            const int blk = d.blk_at[p];
+            const int k0 = d.blev[p] < d.blev[p + 1] ? d.loff[d.blev[p]] : 0;
+            const int k1 = d.blev[p] < d.blev[p + 1] ? d.loff[d.blev[p + 1]] : 0;
+            WsBlock& bk = blocks[p];
+            bk.s0 = static_cast<int>(s0[p]);
+            bk.s1 = static_cast<int>(s0[p + 1]);
+            bk.i0 = static_cast<int>(i0[p]);
+            bk.i1 = static_cast<int>(i0[p + 1]);
+            bk.r0 = blk >= 0 ? other_face[2 * blk] : 0;
+            bk.r1 = blk >= 0 ? other_face[2 * blk + 1] : 0;
+            bk.k0 = k0;
+            bk.nrows = k1 - k0;
+            imap.clear();
+            for (size_t i = 0; i < d.imp[p].size(); ++i) {
+                const int2 x = d.imp[p][i];
+                imports[i0[p] + i] = make_int2(d.face_of[x.x], x.y);
+                imap.emplace_back(x.x, static_cast<int>(i));
+            }
+            std::sort(imap.begin(), imap.end());
+            auto import_slot = [&](int c) {
+                auto it = std::lower_bound(imap.begin(), imap.end(), std::make_pair(c, -1));
+                return max_rows + it->second;
+            };
+            char* base = reinterpret_cast<char*>(words.data()) + s0[p] * 16;
+            const long sbyte0 = s0[p] * 16;
+            long off = 0;
+            long vi = v0[p];
+            for (int g = d.blev[p]; g < d.blev[p + 1]; ++g)
+                for (int c0 = d.loff[g]; c0 < d.loff[g + 1]; c0 += P) {
+                    const int cnt = std::min(P, d.loff[g + 1] - c0);
+                    int w = 0;
+                    for (int k = c0; k < c0 + cnt; ++k) w = std::max(w, entries(d.perm[k]));
+                    w = (w + G - 1) / G * G;
+                    const int rows_b = 16, cols_b = rows_b + a16(cnt * 8), vals_b = cols_b + a16(cnt * w * 4);
+                    const int diag_b = vals_b + a16(cnt * w * sv);
+                    const int rb = diag_b + (upper ? a16(cnt * sv) : 0);
+                    if ((off % kWsRing) + rb > kWsRing) {
+                        const int pad = static_cast<int>(kWsRing - off % kWsRing);
+                        int* h = reinterpret_cast<int*>(base + off);
+                        h[0] = 0; h[1] = 0; h[2] = pad; h[3] = 0;
+                        off += pad;
+                    }
+                    int* h = reinterpret_cast<int*>(base + off);
+                    h[0] = cnt;
+                    h[1] = w;
+                    h[2] = rb;
+                    h[3] = c0 - k0;
+                    int2* rows = reinterpret_cast<int2*>(base + off + rows_b);
+                    int* cols = reinterpret_cast<int*>(base + off + cols_b);
+                    for (int j = 0; j < cnt; ++j) {
+                        const int row = d.perm[c0 + j];
+                        rows[j] = make_int2(row, d.face_of[row]);
+                        const int e0 = upper ? dg[row] + 1 : rp[row];
+                        const int m = entries(row);
+                        for (int q = 0; q < w; ++q) {
+                            int slot = zero, src = -1;
+                            if (q < m) {
+                                const int c = ci[e0 + q];
+                                slot = bo[c] == blk ? d.kinv[c] - k0 : import_slot(c);
+                                src = e0 + q;
+                            }
+                            cols[j * w + q] = slot;
+                            vdst[vi] = static_cast<int>((sbyte0 + off + vals_b + static_cast<long>(j * w + q) * sv) / sv);
+                            vsrc[vi] = src;
+                            ++vi;
+                        }
+                    }
+                    if (upper)
+                        for (int j = 0; j < cnt; ++j) {
+                            vdst[vi] = static_cast<int>((sbyte0 + off + diag_b + static_cast<long>(j) * sv) / sv);
+                            vsrc[vi] = dg[d.perm[c0 + j]];
+                            ++vi;
+                        }
+                    off += rb;
+                }
+        }
+    });
+    wp.stream.resize(words.size() / 4);
+    PILU_HIP(hipMemcpy(wp.stream.get(), words.data(), words.size() * 4, hipMemcpyHostToDevice));
+    wp.vdst.resize(vdst.size());
+    wp.vsrc.resize(vsrc.size());
+    PILU_HIP(hipMemcpy(wp.vdst.get(), vdst.data(), vdst.size() * 4, hipMemcpyHostToDevice));
+    PILU_HIP(hipMemcpy(wp.vsrc.get(), vsrc.data(), vsrc.size() * 4, hipMemcpyHostToDevice));
+    wp.imports.resize(imports.size());
+    PILU_HIP(hipMemcpy(wp.imports.get(), imports.data(), imports.size() * sizeof(int2), hipMemcpyHostToDevice));
+    wp.blocks.resize(sizeof(WsBlock) * static_cast<size_t>(std::max(nblk, 1)));
+    PILU_HIP(hipMemcpy(wp.blocks.get(), blocks.data(), sizeof(WsBlock) * nblk, hipMemcpyHostToDevice));
+    wp.nface = d.face_lo[nblk];
+    wp.face.resize(static_cast<size_t>(std::max(wp.nface, 1)));
+    wp.face_armed = false;
+    wp.nvals = v0[nblk];
+    wp.nstream = s0[nblk];
+    wp.max_imp = max_imp;
+    wp.value_bytes = sv;
+    wp.g = G;
+    wp.version = e->part_version;
+    wp.factors = -1;
+    (void)n;
+}
+
+// LDS of k_sweep_ws for a direction: block x, imports, zero slot, ring.
+size_t ws_lds_bytes(const Ext* e, const WsPack& wp) {
+    return sizeof(double) * static_cast<size_t>((e->max_block_rows + wp.max_imp + 2) & ~1) + kWsRing;
+}
+
+void gather_ws(Ext* e, WsPack& wp) {
+    if (wp.version != e->part_version || wp.nvals <= 0) return;
+    refuse_if_no_alloc(e, "record stream values");
+    hipStream_t s = e->stream;
+    const int grid = 2048;
+    if (wp.value_bytes == 4)
+        k_ws_gather<float, float><<<grid, 256, 0, s>>>(wp.nvals, wp.vdst.get(), wp.vsrc.get(), e->fac32.get(),
+                                                       reinterpret_cast<float*>(wp.stream.get()));
+    else
+        k_ws_gather<double, double><<<grid, 256, 0, s>>>(wp.nvals, wp.vdst.get(), wp.vsrc.get(), e->fac.get(),
+                                                         reinterpret_cast<double*>(wp.stream.get()));
+    PILU_HIP(hipGetLastError());
+    wp.factors = e->factor_version;
+}
+
+// (first, end) face slot of every block id in one direction's slot order.
+std::vector<int> ws_face_ranges(const Ext* e, const WsDir& d) {
+    std::vector<int> out(2 * static_cast<size_t>(e->nblk), 0);
+    for (int p = 0; p < e->nblk; ++p)
+        if (d.blk_at[p] >= 0) {
+            out[2 * d.blk_at[p]] = d.face_lo[p];
+            out[2 * d.blk_at[p] + 1] = d.face_lo[p + 1];
+        }
+    return out;
+}
+
+// Record stream of one direction for G lanes per row.  Face slots do not
+// depend on G, so the other direction's ranges (re-armed by this one) are
+// computed once per partition.
+void build_ws(Ext* e, bool upper, int G) {
+    refuse_if_no_alloc(e, "record streams");
+    if (e->nblk <= 0) throw std::runtime_error("wave-specialised sweep needs a block partition");
+    if (G < 1 || G > 16 || (G & (G - 1))) throw std::runtime_error("wave-specialised sweep: 1..16 lanes per row");
+    host_pattern(e);
+    WsPack& wp = upper ? e->ws_u : e->ws_l;
+    WsPack& other = upper ? e->ws_l : e->ws_u;
+    WsDir d;
+    ws_schedule(e, upper, d, 64 / G);
+    wp.face_rng = ws_face_ranges(e, d);
+    wp.rng_version = e->part_version;
+    if (other.rng_version != e->part_version) {
+        WsDir o;
+        ws_schedule(e, !upper, o, 64);
+        other.face_rng = ws_face_ranges(e, o);
+        other.rng_version = e->part_version;
+    }
+    ws_stream(e, upper, d, other.face_rng, G, wp);
+    if (e->factored) gather_ws(e, wp);
+}
+
+bool ws_current(const Ext* e, const WsPack& wp, int G) {
+    return wp.version == e->part_version && wp.g == G && wp.value_bytes == (e->precision == 32 ? 4 : 8);
+}
+
+void ensure_ws(Ext* e, bool upper, int G) {
+    WsPack& wp = upper ? e->ws_u : e->ws_l;
+    if (!ws_current(e, wp, G)) build_ws(e, upper, G);
+    if (e->factored && wp.factors != e->factor_version) gather_ws(e, wp);
+}
+
+bool ws_fits(Ext* e, int G) {
+    try {
+        ensure_ws(e, false, G);
+        ensure_ws(e, true, G);
+    } catch (const std::runtime_error&) {
+        return false;
+    }
+    return ws_lds_bytes(e, e->ws_l) <= 150 * 1024 && ws_lds_bytes(e, e->ws_u) <= 150 * 1024;
+}
+
+template <int G, int S, typename T, bool UPPER>
+void sweep_ws_one(Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+    WsPack& wp = UPPER ? e->ws_u : e->ws_l;
+    WsPack& other = UPPER ? e->ws_l : e->ws_u;
+    // the other direction's face buffer is re-armed here when it sweeps in
+    // this mode over the same partition
+    const bool other_ws = (UPPER ? e->mode_l : e->mode_u) == 5 && other.version == e->part_version &&
+                          other.face.get() != nullptr;
+    const size_t lds = ws_lds_bytes(e, wp);
+    if (lds > 150 * 1024) throw std::runtime_error("wave-specialised sweep: block and imports exceed the LDS");
+    if (!wp.stream.get() || !wp.blocks.get() || !wp.imports.get() || !wp.face.get() || wp.factors != e->factor_version)
+        throw std::runtime_error("wave-specialised sweep: record streams missing (factorize before sweeping)");
+    hipStream_t s = e->stream;
+    if (!wp.face_armed)
+        k_fill_u64<<<grid_for(std::max(wp.nface, 1), 256 * 8), 256, 0, s>>>(std::max(wp.nface, 1),
+                                                                            reinterpret_cast<uint64_t*>(wp.face.get()),
+                                                                            kPending);
+    auto kern = k_sweep_ws<G, S, T, UPPER>;
+    PILU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(lds)));
+    kern<<<e->nblk, kWsThreads, lds, s>>>(reinterpret_cast<const WsBlock*>(wp.blocks.get()), wp.stream.get(),
+                                          wp.imports.get(), UPPER ? e->bperm_u.get() : e->bperm_l.get(), b, out, reset,
+                                          wp.face.get(), other_ws ? other.face.get() : nullptr, gate, ctr,
+                                          e->err.get() + kErrSweep, watchdog_ticks(e->watchdog_s), e->max_block_rows,
+                                          wp.max_imp);
+    wp.face_armed = false;
+    if (other_ws) other.face_armed = true;
+}
+
+int effective_lanes(const Ext* e, bool upper);
+
+// Lanes per row of the wave-specialised sweep: the configured count, at
+// most 16 (records hold 64 / G rows).
+int ws_lanes(int lanes) { return std::min(std::max(lanes, 1), 16); }
+
+// Builds what the configured sweep kinds need (packs, record streams), so
+// that the first apply allocates nothing more.
+void prepare_modes(Ext* e) {
+    const bool pp = (e->mode_l >= 2 && e->mode_l <= 3) || (e->mode_u >= 2 && e->mode_u <= 3);
+    if (pp) ensure_packs(e);
+    if (e->mode_l == 5) ensure_ws(e, false, ws_lanes(effective_lanes(e, false)));
+    if (e->mode_u == 5) ensure_ws(e, true, ws_lanes(effective_lanes(e, true)));
+}
+
+
+template <typename T, bool UPPER>
+void sweep_ws_dispatch(int lanes, Ext* e, const double* b, double* out, double* reset, const int* gate, int* ctr) {
+    lanes = ws_lanes(lanes);
+    ensure_ws(e, UPPER, lanes);
+    switch (lanes) {
+        case 1: sweep_ws_one<1, 8, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 2: sweep_ws_one<2, 4, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 4: sweep_ws_one<4, 2, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        case 8: sweep_ws_one<8, 1, T, UPPER>(e, b, out, reset, gate, ctr); break;
+        default: sweep_ws_one<16, 1, T, UPPER>(e, b, out, reset, gate, ctr); break;
+    }
+}
+
 template <int G, typename T, bool UPPER>
 void sweep_one(Ext* e, const T* val, const double* b, double* out, double* reset, const int* gate, int* ctr) {
     constexpr int rows_per_block = (64 / G) * kWavesPerBlock;
@@ -934,7 +1308,11 @@ void sweep_dispatch(int lanes, Ext* e, const T* val, const double* b, double* ou
         }
         return;
     }
-    if (mode >= 2 && e->nblk > 0) {
+    if (mode == 5 && e->nblk > 0) {
+        sweep_ws_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
+        return;
+    }
+    if (mode >= 2 && mode <= 3 && e->nblk > 0) {
         ensure_packs(e);
         sweep_pp_dispatch<T, UPPER>(lanes, e, b, out, reset, gate, ctr);
         return;
@@ -1010,6 +1388,8 @@ bool run_factor(Ext* e) {
     ++e->factor_version;
     gather_pack(e, e->pack_l, false);
     gather_pack(e, e->pack_u, true);
+    if (ws_current(e, e->ws_l, e->ws_l.g)) gather_ws(e, e->ws_l);
+    if (ws_current(e, e->ws_u, e->ws_u.g)) gather_ws(e, e->ws_u);
     return (err & 1) == 0;
 }
 
@@ -1027,24 +1407,46 @@ void show_algorithm(const lljbash::IluSolver* solver) {
     if (v && *v && std::strcmp(v, "0") != 0) std::printf("%s\n", solver->DescribeSchedule().c_str());
 }
 
+// Sweep kinds whose output must hold kPending in every slot on entry
+// (value-flag synchronisation through the output vector).
+bool flags_in_output(int mode) { return mode >= 0 && mode <= 3; }
+
+// The forward sweep b -> y, then the backward sweep y -> x.  Each sweep
+// re-arms the other one's output where the other one synchronises through
+// it: the forward sweep re-arms x (after reading b, which may alias x),
+// the backward sweep re-arms y for the next forward sweep.  y must hold
+// kPending on entry when the forward sweep needs it (y_armed tracks that
+// for the solver's own y).  ev (optional): events recorded before, between
+// and after the two sweeps.
+void sweep_pair(Ext* e, const double* d_b, double* d_x, double* y, const int* gate, int* ctr, hipEvent_t* ev) {
+    hipStream_t s = e->stream;
+    const int gl = effective_lanes(e, false), gu = effective_lanes(e, true);
+    double* reset_l = flags_in_output(e->mode_u) ? d_x : nullptr;
+    double* reset_u = flags_in_output(e->mode_l) ? y : nullptr;
+    if (ev) PILU_HIP(hipEventRecord(ev[0], s));
+    if (e->precision == 32) sweep_dispatch<float, false>(gl, e, e->fac32.get(), d_b, y, reset_l, gate, ctr);
+    else sweep_dispatch<double, false>(gl, e, e->fac.get(), d_b, y, reset_l, gate, ctr);
+    if (ev) PILU_HIP(hipEventRecord(ev[1], s));
+    if (e->precision == 32) sweep_dispatch<float, true>(gu, e, e->fac32.get(), y, d_x, reset_u, gate, ctr + 2);
+    else sweep_dispatch<double, true>(gu, e, e->fac.get(), y, d_x, reset_u, gate, ctr + 2);
+    if (ev) PILU_HIP(hipEventRecord(ev[2], s));
+    PILU_HIP(hipGetLastError());
+}
+
+// Fills the solver's y with kPending when the forward sweep synchronises
+// through it and the last backward sweep did not re-arm it.
+void arm_y(Ext* e) {
+    if (flags_in_output(e->mode_l) && !e->y_armed)
+        k_fill_u64<<<grid_for(e->n, 256 * 8), 256, 0, e->stream>>>(e->n, reinterpret_cast<uint64_t*>(e->y.get()),
+                                                                    kPending);
+}
+
 void ilu_apply(Ext* e, const double* d_b, double* d_x, const int* gate) {
     need_setup(e);
     if (!e->factored) throw std::runtime_error("IluSolver: Factorize has not been called");
-    hipStream_t s = e->stream;
-    if (!e->y_armed) {
-        k_fill_u64<<<grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(e->y.get()), kPending);
-        e->y_armed = true;
-    }
-    const int gl = effective_lanes(e, false), gu = effective_lanes(e, true);
-    int* ctr = e->ctr.get() + 4;
-    if (e->precision == 32) {
-        sweep_dispatch<float, false>(gl, e, e->fac32.get(), d_b, e->y.get(), d_x, gate, ctr);
-        sweep_dispatch<float, true>(gu, e, e->fac32.get(), e->y.get(), d_x, e->y.get(), gate, ctr + 2);
-    } else {
-        sweep_dispatch<double, false>(gl, e, e->fac.get(), d_b, e->y.get(), d_x, gate, ctr);
-        sweep_dispatch<double, true>(gu, e, e->fac.get(), e->y.get(), d_x, e->y.get(), gate, ctr + 2);
-    }
-    PILU_HIP(hipGetLastError());
+    arm_y(e);
+    sweep_pair(e, d_b, d_x, e->y.get(), gate, e->ctr.get() + 4, nullptr);
+    e->y_armed = flags_in_output(e->mode_l);
 }
 
 // ---------------------------------------------------------------------
@@ -1102,15 +1504,7 @@ void ilu_apply_no_alloc(Ext* e, const double* d_b, double* d_x) {
     // zero (fill kernels, so the sequence can also be captured in a graph)
     k_fill_u64<<<grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(y), kPending);
     k_fill_u64<<<1, 64, 0, s>>>(16, reinterpret_cast<uint64_t*>(prog), 0);
-    const int gl = effective_lanes(e, false), gu = effective_lanes(e, true);
-    if (e->precision == 32) {
-        sweep_dispatch<float, false>(gl, e, e->fac32.get(), d_b, y, d_x, nullptr, ctr);
-        sweep_dispatch<float, true>(gu, e, e->fac32.get(), y, d_x, y, nullptr, ctr + 2);
-    } else {
-        sweep_dispatch<double, false>(gl, e, e->fac.get(), d_b, y, d_x, nullptr, ctr);
-        sweep_dispatch<double, true>(gu, e, e->fac.get(), y, d_x, y, nullptr, ctr + 2);
-    }
-    PILU_HIP(hipGetLastError());
+    sweep_pair(e, d_b, d_x, y, nullptr, ctr, nullptr);
 }
 
 // Schedule of independent diagonal blocks given by block_of (the systems of
@@ -1275,18 +1669,12 @@ void time_sweeps(Ext* e, int lanes, const double* b, double* x, int reps, hipEve
     e->lanes_l = e->lanes_u = lanes;
     ilu_apply(e, b, x, nullptr);  // warm-up; also arms the buffers
     tl = tu = 0.0;
+    (void)s;
+    (void)ctr;
     for (int r = 0; r < reps; ++r) {
-        PILU_HIP(hipEventRecord(ev[0], s));
-        if (e->precision == 32)
-            sweep_dispatch<float, false>(lanes, e, e->fac32.get(), b, e->y.get(), x, nullptr, ctr);
-        else
-            sweep_dispatch<double, false>(lanes, e, e->fac.get(), b, e->y.get(), x, nullptr, ctr);
-        PILU_HIP(hipEventRecord(ev[1], s));
-        if (e->precision == 32)
-            sweep_dispatch<float, true>(lanes, e, e->fac32.get(), e->y.get(), x, e->y.get(), nullptr, ctr + 2);
-        else
-            sweep_dispatch<double, true>(lanes, e, e->fac.get(), e->y.get(), x, e->y.get(), nullptr, ctr + 2);
-        PILU_HIP(hipEventRecord(ev[2], s));
+        arm_y(e);
+        sweep_pair(e, b, x, e->y.get(), nullptr, e->ctr.get() + 4, ev);
+        e->y_armed = flags_in_output(e->mode_l);
         PILU_HIP(hipEventSynchronize(ev[2]));
         float a = 0.f, c = 0.f;
         PILU_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
@@ -1314,6 +1702,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     static const int kLanes[7] = {1, 2, 4, 8, 16, 32, 64};
     static const int kBlkLanes[4] = {4, 8, 16, 32};
     static const int kPackLanes[4] = {2, 4, 8, 16};
+    static const int kWsLanes[3] = {1, 2, 4};
     struct Part { int blk, tx, ty, tz; };
     std::vector<Part> parts;
     for (int blk : {4096, 8192, 16384})
@@ -1399,6 +1788,17 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
                 if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = mode; }
             }
         }
+        // wave-specialised sweep over record streams (rows of up to a few
+        // tens of entries; blocks whose x and imports fit the LDS)
+        e->mode_l = e->mode_u = 5;
+        for (int g : kWsLanes) {
+            if (!pilu::ws_fits(e, g)) continue;
+            double tl, tu;
+            pilu::time_sweeps(e, g, b.get(), x.get(), reps, ev, tl, tu);
+            record(5, pt, g, tl, tu);
+            if (tl < bb[0]) { bb[0] = tl; bl[0] = g; bm[0] = 5; }
+            if (tu < bb[1]) { bb[1] = tu; bl[1] = g; bm[1] = 5; }
+        }
         const double tot = std::min(bb[0], lev_best[0]) + std::min(bb[1], lev_best[1]);
         if (tot < best_total) {
             best_total = tot;
@@ -1420,20 +1820,20 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
     e->mode_u = pick_mode[1];
     e->lanes_l = pick_lanes[0];
     e->lanes_u = pick_lanes[1];
-    if ((pick_mode[0] >= 2 && pick_mode[0] <= 3) || (pick_mode[1] >= 2 && pick_mode[1] <= 3)) pilu::ensure_packs(e);
+    pilu::prepare_modes(e);
     pilu::show_algorithm(this);
     return rows;
 }
 
 void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
     Ext* e = pilu::need_setup(ext_);
-    if (mode_l < 0 || mode_l > 4 || mode_u < 0 || mode_u > 4)
-        throw std::runtime_error("sweep mode is 0, 1, 2, 3 or 4");
-    auto blocked = [](int m) { return m >= 1 && m <= 3; };
+    if (mode_l < 0 || mode_l > 5 || mode_u < 0 || mode_u > 5)
+        throw std::runtime_error("sweep mode is 0, 1, 2, 3, 4 or 5");
+    auto blocked = [](int m) { return (m >= 1 && m <= 3) || m == 5; };
     if (blocked(mode_l) || blocked(mode_u)) pilu::use_blocks(e, blk_rows);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
-    if ((mode_l >= 2 && mode_l <= 3) || (mode_u >= 2 && mode_u <= 3)) pilu::ensure_packs(e);
+    pilu::prepare_modes(e);
     pilu::show_algorithm(this);
 }
 
@@ -1448,12 +1848,12 @@ void IluSolver::SetGridShape(int nx, int ny, int nz) {
 
 void IluSolver::SetSweepTiles(int mode_l, int mode_u, int tx, int ty, int tz) {
     Ext* e = pilu::need_setup(ext_);
-    if (mode_l == 1 || mode_u == 1 || mode_l < 0 || mode_l > 3 || mode_u < 0 || mode_u > 3)
-        throw std::runtime_error("tiled sweeps use mode 0, 2 or 3");
+    auto ok = [](int m) { return m == 0 || m == 2 || m == 3 || m == 5; };
+    if (!ok(mode_l) || !ok(mode_u)) throw std::runtime_error("tiled sweeps use mode 0, 2, 3 or 5");
     pilu::use_tiles(e, tx, ty, tz);
     e->mode_l = mode_l;
     e->mode_u = mode_u;
-    if (mode_l >= 2 || mode_u >= 2) pilu::ensure_packs(e);
+    pilu::prepare_modes(e);
     pilu::show_algorithm(this);
 }
 
