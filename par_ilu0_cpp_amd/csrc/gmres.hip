@@ -391,6 +391,7 @@ struct lljbash::PreconditionedGmres::Ext {
     std::vector<hipEvent_t> tev;     // timing events (pairs)
     int hist_len = 0;
     double true_res = -1.0;
+    std::vector<double> hv, hz;      // host vectors of a host-side preconditioner
     ~Ext() {
         if (snap) (void)hipHostFree(snap);
         for (auto& e : snap_ev) if (e) (void)hipEventDestroy(e);
@@ -496,13 +497,20 @@ struct TimedPair {
     int kind;  // 0 = mv, 1 = precon
 };
 
-std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& prm, GmresStat& st, int n,
-                                long nnz, const int* rp, const int* ci, const double* val, const double* rhs,
-                                double* sol) {
-    if (!ilu || !ilu->DeviceState()) return {false, 0};
-    IluSolver::Ext* ie = ilu->DeviceState();
-    if (ie->n != n) throw std::runtime_error("GMRES: matrix size differs from the preconditioner's");
-    hipStream_t s = pilu::ilu_stream(ie);
+// precon (used when ilu is null): M^-1 on host vectors, called once per
+// Arnoldi step with the step's basis vector copied to the host.
+std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const std::function<void(const double*, double*)>* precon,
+                                const GmresParameters& prm, GmresStat& st, int n, long nnz, const int* rp,
+                                const int* ci, const double* val, const double* rhs, double* sol) {
+    const bool host_precon = !ilu && precon && *precon;
+    if (!host_precon && (!ilu || !ilu->DeviceState())) return {false, 0};
+    IluSolver::Ext* ie = host_precon ? nullptr : ilu->DeviceState();
+    if (ie && ie->n != n) throw std::runtime_error("GMRES: matrix size differs from the preconditioner's");
+    hipStream_t s = ie ? pilu::ilu_stream(ie) : nullptr;
+    if (host_precon) {
+        e->hv.resize(static_cast<size_t>(n));
+        e->hz.resize(static_cast<size_t>(n));
+    }
     const int m = prm.krylov_subspace_dimension;
     const int im1 = m + 1;
     if (m < 1) throw std::runtime_error("GMRES: krylov_subspace_dimension must be >= 1");
@@ -606,7 +614,20 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
             double* zi = z + static_cast<long>(i) * n;
             pilu::k_step_gate<<<1, 1, 0, s>>>(dctl, i);
             t_begin(1);
-            pilu::ilu_apply(ie, vi, zi, step_gate);
+            if (host_precon) {
+                // the step runs only while the cycle is active (the gate the
+                // device kernels read)
+                pilu::GmresCtl hc{};
+                PILU_HIP(hipMemcpyAsync(&hc, dctl, sizeof(hc), hipMemcpyDeviceToHost, s));
+                PILU_HIP(hipStreamSynchronize(s));
+                if (hc.step_active) {
+                    PILU_HIP(hipMemcpy(e->hv.data(), vi, sizeof(double) * n, hipMemcpyDeviceToHost));
+                    (*precon)(e->hv.data(), e->hz.data());
+                    PILU_HIP(hipMemcpy(zi, e->hz.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+                }
+            } else {
+                pilu::ilu_apply(ie, vi, zi, step_gate);
+            }
             t_end();
             t_begin(0);
             spmv(lanes, s, n, rp, ci, val, 1.0, zi, 0.0, nullptr, vi1, vv, false, part0, step_gate);
@@ -650,7 +671,7 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
     spmv(lanes, s, n, rp, ci, val, -1.0, sol, 1.0, rhs, e->scratch.get(), nullptr, true, part0, nullptr);
     pilu::k_sqrt_sum<<<1, pilu::kVecThreads, 0, s>>>(part0, part1);
     PILU_HIP(hipMemcpyAsync(&e->true_res, part1, sizeof(double), hipMemcpyDeviceToHost, s));
-    pilu::check_sweep_errors(ie);
+    if (ie) pilu::check_sweep_errors(ie);
     if (pbuf) {
         int werr = 0;
         PILU_HIP(hipMemcpyAsync(&werr, mgs_err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -671,11 +692,11 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const GmresParameters& 
 
 }  // namespace
 
-PreconditionedGmres::~PreconditionedGmres() { delete ext_; }
+PreconditionedGmres::~PreconditionedGmres() { delete intermediate_.ext; }
 
 std::pair<bool, int> PreconditionedGmres::Solve(const CsrMatrix* mat, double* rhs, double* sol) {
-    if (!ext_) ext_ = new Ext;
-    Ext* e = ext_;
+    if (!intermediate_.ext) intermediate_.ext = new Ext;
+    Ext* e = intermediate_.ext;
     const int n = mat->size;
     const long nnz = mat->row_ptr[n];
     const auto t0 = std::chrono::steady_clock::now();
@@ -690,28 +711,31 @@ std::pair<bool, int> PreconditionedGmres::Solve(const CsrMatrix* mat, double* rh
     PILU_HIP(hipMemcpy(e->rhs.get(), rhs, sizeof(double) * n, hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(e->sol.get(), sol, sizeof(double) * n, hipMemcpyHostToDevice));
     stat_.total_init_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    auto r = solve_core(e, ilu_, param_, stat_, n, nnz, e->arp.get(), e->aci.get(), e->aval.get(), e->rhs.get(),
-                        e->sol.get());
+    auto r = solve_core(e, intermediate_.ilu, &precon_, param_, stat_, n, nnz, e->arp.get(), e->aci.get(),
+                        e->aval.get(), e->rhs.get(), e->sol.get());
     PILU_HIP(hipMemcpy(sol, e->sol.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
     return r;
 }
 
 std::pair<bool, int> PreconditionedGmres::SolveDevice(const CsrMatrix* d_mat, const double* d_rhs, double* d_sol) {
-    if (!ext_) ext_ = new Ext;
+    if (!intermediate_.ext) intermediate_.ext = new Ext;
     const int n = d_mat->size;
     int h_last = 0;
     PILU_HIP(hipMemcpy(&h_last, d_mat->row_ptr + n, sizeof(int), hipMemcpyDeviceToHost));
     const long nnz = h_last;
-    return solve_core(ext_, ilu_, param_, stat_, n, nnz, d_mat->row_ptr, d_mat->col_idx, d_mat->value, d_rhs,
-                      d_sol);
+    return solve_core(intermediate_.ext, intermediate_.ilu, &precon_, param_, stat_, n, nnz, d_mat->row_ptr,
+                      d_mat->col_idx, d_mat->value, d_rhs, d_sol);
 }
 
-double PreconditionedGmres::LastTrueResidual() const { return ext_ ? ext_->true_res : -1.0; }
+double PreconditionedGmres::LastTrueResidual() const {
+    return intermediate_.ext ? intermediate_.ext->true_res : -1.0;
+}
 
 int PreconditionedGmres::ResidualHistory(double* out, int cap) const {
-    if (!ext_ || ext_->hist_len <= 0) return 0;
-    const int k = std::min(cap, ext_->hist_len);
-    if (k > 0) PILU_HIP(hipMemcpy(out, ext_->hist.get(), sizeof(double) * k, hipMemcpyDeviceToHost));
+    const Ext* e = intermediate_.ext;
+    if (!e || e->hist_len <= 0) return 0;
+    const int k = std::min(cap, e->hist_len);
+    if (k > 0) PILU_HIP(hipMemcpy(out, e->hist.get(), sizeof(double) * k, hipMemcpyDeviceToHost));
     return k;
 }
 
