@@ -61,6 +61,11 @@ void check_sweep_errors(lljbash::IluSolver::Ext* e) {
         e->y_armed = false;
         throw std::runtime_error("triangular sweep: dependency wait timed out (watchdog)");
     }
+    if (v & kErrStream) {
+        PILU_HIP(hipMemsetAsync(e->err.get() + kErrSweep, 0, sizeof(int), e->stream));
+        e->y_armed = false;
+        throw std::runtime_error("wave-specialised sweep: record stream row or face id out of range");
+    }
 }
 
 double* device_scratch();
@@ -1086,7 +1091,7 @@ void sweep_ws_one(Ext* e, const double* b, double* out, double* reset, const int
                                           wp.imports.get(), UPPER ? e->bperm_u.get() : e->bperm_l.get(), b, out, reset,
                                           wp.face.get(), other_ws ? other.face.get() : nullptr, gate, ctr,
                                           e->err.get() + kErrSweep, watchdog_ticks(e->watchdog_s), e->max_block_rows,
-                                          wp.max_imp);
+                                          wp.max_imp, e->n, wp.nface);
     wp.face_armed = false;
     if (other_ws) other.face_armed = true;
 }

@@ -14,8 +14,9 @@
 //     value-flags.
 //   * wave 1 streams the block's record stream -- headers, row ids,
 //     columns already translated to LDS slots, values, diagonal -- from HBM
-//     into the LDS ring in 1 KiB chunks, 16 chunks in flight, flow-
-//     controlled by two LDS counters (bytes loaded / bytes consumed).
+//     into the LDS ring in groups of eight 1 KiB chunks, two groups in
+//     flight, flow-controlled by two LDS counters (bytes loaded / bytes
+//     consumed).
 //   * wave 2 imports: it polls the face buffer for the block's out-of-block
 //     dependencies, in order of first use and at most kWsLook records ahead
 //     of the solver, and writes each value into its LDS slot as a
@@ -41,9 +42,10 @@ namespace pilu {
 constexpr int kWsThreads = 192;   // solver, loader and importer waves
 constexpr int kWsChunk = 1024;    // bytes the loader moves per step (64 lanes x 16 B)
 constexpr int kWsRing = 16384;    // LDS ring of the record stream (bytes)
-constexpr int kWsInflight = 16;   // loader chunks in flight
+constexpr int kWsGroup = 8;       // chunks the loader writes per ring-space check (two groups in flight)
 constexpr int kWsImp = 4;         // imports each importer lane polls at once
 constexpr int kWsLook = 8;        // records ahead of the solver the importer polls
+constexpr int kErrStream = 1 << 9;  // error word: a record stream row or face id out of range
 
 struct WsBlock {
     int s0, s1;     // record stream of the block, in 16-byte units (a multiple of kWsChunk bytes)
@@ -80,113 +82,229 @@ __device__ __forceinline__ bool ws_wait_loaded(const int* ctl, int need, const W
     return true;
 }
 
-// Wave 0: walk the block's records.
+// One record as the solver holds it: header, this lane's row data and the
+// columns and values of its first S entries (read from the ring one record
+// ahead of their use).
+template <int S, typename T>
+struct WsRec {
+    int cnt, w, rbytes, slot0, off, vals_b;
+    int2 rf;
+    double dv, b;
+    int c[S];
+    T vr[S];
+};
+
+// Reads the next record at stream offset `off` (skipping ring padding);
+// false at the block's end or on a watchdog expiry (timed_out set).
 template <int G, int S, typename T, bool UPPER>
-__device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* xl, double* out, double* face,
-                         const Watchdog& wd, bool& timed_out) {
+__device__ __forceinline__ bool ws_fetch(int off, int bytes, const char* ring, const int* ctl, const double* xl,
+                                         int zero_slot, const Watchdog& wd, bool& timed_out, WsRec<S, T>& r) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane / G, t = lane & (G - 1);
+    int cnt = 0, w = 0, rbytes = 0, slot0 = 0, base = 0;
+    while (true) {
+        if (off >= bytes) return false;
+        if (!ws_wait_loaded(ctl, off + 16, wd)) { timed_out = true; return false; }
+        base = off & (kWsRing - 1);
+        const int4 h = *reinterpret_cast<const int4*>(ring + base);
+        cnt = __builtin_amdgcn_readfirstlane(h.x);
+        w = __builtin_amdgcn_readfirstlane(h.y);
+        rbytes = __builtin_amdgcn_readfirstlane(h.z);
+        slot0 = __builtin_amdgcn_readfirstlane(h.w);
+        if (cnt > 0) break;
+        if (rbytes <= 0) return false;  // padding up to the block's end
+        off += rbytes;                   // padding up to a ring boundary
+    }
+    if (!ws_wait_loaded(ctl, off + rbytes, wd)) { timed_out = true; return false; }
+    const int rows_b = base + 16;
+    const int cols_b = rows_b + ws_align16(cnt * 8);
+    const int vals_b = cols_b + ws_align16(cnt * w * 4);
+    const int diag_b = vals_b + ws_align16(cnt * w * static_cast<int>(sizeof(T)));
+    const int jj = j < cnt ? j : 0;
+    r.cnt = cnt;
+    r.w = w;
+    r.rbytes = rbytes;
+    r.slot0 = slot0;
+    r.off = off;
+    r.vals_b = vals_b + jj * w * static_cast<int>(sizeof(T));
+    r.rf = reinterpret_cast<const int2*>(ring + rows_b)[jj];
+    r.dv = UPPER ? static_cast<double>(reinterpret_cast<const T*>(ring + diag_b)[jj]) : 1.0;
+    r.b = xl[slot0 + jj];
+    const int* cp = reinterpret_cast<const int*>(ring + cols_b) + jj * w;
+    const T* vp = reinterpret_cast<const T*>(ring + vals_b) + jj * w;
+    // branch-free: terms past the row's width read its last entry and are
+    // turned into (zero slot, 0), which adds -0.0
+    const int wl = w > 0 ? w - 1 : 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.c[s] = w > 0 ? cp[t + s * G < w ? t + s * G : wl] : zero_slot;
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.vr[s] = w > 0 ? vp[t + s * G < w ? t + s * G : wl] : T(0);
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (!(s * G < w)) {
+            r.c[s] = zero_slot;
+            r.vr[s] = T(0);
+        }
+    return true;
+}
+
+// Wave 0: walk the block's records, two stages deep: while record r's x
+// are read and summed, record r+1's header, row data, columns and values
+// are read from the ring, so only the x reads (which depend on the rows
+// just solved) stay on the critical path.
+//
+// Ring flow control: the solver holds at most one record's ring bytes
+// unconsumed while it waits for the loader.  A record whose entries all sit
+// in registers after ws_fetch (w <= S * G) is released before record r+1
+// is fetched; a longer row's record is read again by the tail loop, so it
+// is released after its rows are solved and record r+1 is fetched then.
+// Holding r while waiting for r+1 could deadlock: the loader refills a
+// ring half only once the half before it is consumed, and two records can
+// together exceed the free half.
+template <int G, int S, typename T, bool UPPER>
+__device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* xl, int zero_slot, double* out,
+                         double* face, int n, int nface, const Watchdog& wd, bool& timed_out, bool& bad) {
     const int lane = threadIdx.x & 63;
     const int j = lane / G;
     const int t = lane & (G - 1);
     const int bytes = (bk.s1 - bk.s0) * 16;
-    int off = 0;
+    WsRec<S, T> cur, nxt;
+    bool have = ws_fetch<G, S, T, UPPER>(0, bytes, ring, ctl, xl, zero_slot, wd, timed_out, cur);
     int rec = 0;
-    while (off < bytes) {
-        if (!ws_wait_loaded(ctl, off + 16, wd)) { timed_out = true; break; }
-        const int base = off & (kWsRing - 1);
-        const int4 h = *reinterpret_cast<const int4*>(ring + base);
-        const int cnt = __builtin_amdgcn_readfirstlane(h.x);
-        const int w = __builtin_amdgcn_readfirstlane(h.y);
-        const int rbytes = __builtin_amdgcn_readfirstlane(h.z);
-        const int slot0 = __builtin_amdgcn_readfirstlane(h.w);
-        if (cnt <= 0) {
-            // padding: up to a ring boundary (bytes > 0) or to the block's end
-            if (rbytes <= 0) break;
-            off += rbytes;
-            lds_release();
-            if (lane == 0) lds_store_i(&ctl[1], off);
-            continue;
+    while (have) {
+        const bool act = j < cur.cnt;
+        const bool tail = cur.w > S * G;
+        const int next_off = cur.off + cur.rbytes;
+        if (!tail) {
+            lds_release();  // this record's ring reads have returned
+            if (lane == 0) lds_store_i(&ctl[1], next_off);
         }
-        if (!ws_wait_loaded(ctl, off + rbytes, wd)) { timed_out = true; break; }
-        const int rows_b = base + 16;
-        const int cols_b = rows_b + ws_align16(cnt * 8);
-        const int vals_b = cols_b + ws_align16(cnt * w * 4);
-        const int diag_b = vals_b + ws_align16(cnt * w * static_cast<int>(sizeof(T)));
-        const bool act = j < cnt;
-        const int jj = act ? j : 0;
-        const int* cp = reinterpret_cast<const int*>(ring + cols_b) + jj * w;
-        const T* vp = reinterpret_cast<const T*>(ring + vals_b) + jj * w;
-        double acc = 0.0;
-        if (act && t == 0) acc = xl[slot0 + j];
-        if (act) {
-            auto term = [&](int q) {
-                const int c = cp[q];
-                const double v = static_cast<double>(vp[q]);
-                uint64_t bits = lds_load_u64(xl + c);
+        // x of this record's first S entries: every read at once
+        uint64_t xb[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) xb[s] = lds_load_u64(xl + cur.c[s]);
+        // the next record, while those reads are in flight
+        bool more = false;
+        if (!tail) more = ws_fetch<G, S, T, UPPER>(next_off, bytes, ring, ctl, xl, zero_slot, wd, timed_out, nxt);
+        bool pend = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) pend |= xb[s] == kPending;
+        if (pend) {
+            // an import not there yet: wait for it
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                while (xb[s] == kPending) {
+                    if (timed_out || wd.expired()) { timed_out = true; xb[s] = kQuietNan; break; }
+                    __builtin_amdgcn_s_sleep(0);
+                    xb[s] = lds_load_u64(xl + cur.c[s]);
+                }
+        }
+        double acc = act && t == 0 ? cur.b : 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc = fma(-__longlong_as_double(xb[s]), static_cast<double>(cur.vr[s]), acc);
+        if (act && tail) {
+            // entries past the first S per lane (rows longer than S * G)
+            const int rows_b = (cur.off & (kWsRing - 1)) + 16;
+            const int* cp = reinterpret_cast<const int*>(ring + rows_b + ws_align16(cur.cnt * 8)) + j * cur.w;
+            const T* vp = reinterpret_cast<const T*>(ring + cur.vals_b);
+            for (int q = t + S * G; q < cur.w; q += G) {
+                const int cq = cp[q];
+                uint64_t bits = lds_load_u64(xl + cq);
                 while (bits == kPending) {
                     if (timed_out || wd.expired()) { timed_out = true; bits = kQuietNan; break; }
                     __builtin_amdgcn_s_sleep(0);
-                    bits = lds_load_u64(xl + c);
+                    bits = lds_load_u64(xl + cq);
                 }
-                acc = fma(-__longlong_as_double(bits), v, acc);
-            };
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-                if (s * G < w) term(t + s * G);
-            for (int q = t + S * G; q < w; q += G) term(q);
+                acc = fma(-__longlong_as_double(bits), static_cast<double>(vp[q]), acc);
+            }
         }
         const double sum = group_sum<G>(acc);
         if (act && t == 0) {
-            const int2 rf = reinterpret_cast<const int2*>(ring + rows_b)[j];
-            double res = sum;
-            if (UPPER) res = sum / static_cast<double>(reinterpret_cast<const T*>(ring + diag_b)[j]);
+            const double res = UPPER ? sum / cur.dv : sum;
             uint64_t rb = __double_as_longlong(res);
             if (res != res) rb = kQuietNan;
-            lds_store_u64(xl + slot0 + j, rb);
-            out[rf.x] = __longlong_as_double(rb);
-            if (rf.y >= 0) publish_raw(reinterpret_cast<uint64_t*>(face) + rf.y, rb);
+            lds_store_u64(xl + cur.slot0 + j, rb);
+            // row ids come from the stream the host built; an id out of
+            // range means a corrupted stream and is reported, not stored
+            if (static_cast<unsigned>(cur.rf.x) < static_cast<unsigned>(n)) out[cur.rf.x] = __longlong_as_double(rb);
+            else bad = true;
+            if (static_cast<unsigned>(cur.rf.y) < static_cast<unsigned>(nface))
+                publish_raw(reinterpret_cast<uint64_t*>(face) + cur.rf.y, rb);
+            else if (cur.rf.y != -1) bad = true;
         }
-        off += rbytes;
         ++rec;
-        lds_release();  // this record's ring reads have returned
-        if (lane == 0) {
-            lds_store_i(&ctl[1], off);
-            lds_store_i(&ctl[2], rec);
+        if (tail) {
+            lds_release();  // the tail loop's ring reads have returned
+            if (lane == 0) lds_store_i(&ctl[1], next_off);
         }
+        if (lane == 0) lds_store_i(&ctl[2], rec);
+        if (tail) more = ws_fetch<G, S, T, UPPER>(next_off, bytes, ring, ctl, xl, zero_slot, wd, timed_out, nxt);
+        cur = nxt;
+        have = more;
     }
-    if (lane == 0) lds_store_i(&ctl[2], 1 << 30);  // the importer may stop waiting for progress
+    if (lane == 0) {
+        lds_store_i(&ctl[1], bytes);
+        lds_store_i(&ctl[2], 1 << 30);  // the importer may stop waiting for progress
+    }
 }
 
-// Wave 1: stream the block's records into the ring.
-__device__ __forceinline__ void ws_load(const WsBlock& bk, const uint4* __restrict__ stream, uint4* ring, int* ctl,
-                                        const Watchdog& wd, bool& timed_out) {
+// Wave 1: stream the block's records into the ring.  The ring holds two
+// groups of kWsGroup chunks; the loads of group g+2 are issued as group g
+// is written, so two groups are always in flight.  Inside a group there is
+// no control flow, which lets the compiler wait for each chunk with an
+// exact count of the younger loads instead of draining them all.
+//
+// The loader has no watchdog of its own and never writes a ring half
+// before the solver has consumed it: its only wait is for the consumed
+// count, which the solver sets to the whole stream when it leaves (after
+// its last record or on its own watchdog expiry), so the wait always ends.
+__device__ __forceinline__ void ws_load(const WsBlock& bk, const uint4* __restrict__ stream, uint4* ring, int* ctl) {
+    constexpr int kGroupBytes = kWsGroup * kWsChunk;
+    static_assert(2 * kGroupBytes == kWsRing, "the ring holds two groups");
     const int lane = threadIdx.x & 63;
     const int nch = (bk.s1 - bk.s0) / (kWsChunk / 16);
     if (nch <= 0) return;
+    const int ngroups = (nch + kWsGroup - 1) / kWsGroup;
     const long src0 = bk.s0;
     auto src = [&](int c) -> uint4 {
         const int cc = c < nch ? c : nch - 1;
         return stream[src0 + static_cast<long>(cc) * (kWsChunk / 16) + lane];
     };
-    uint4 data[kWsInflight];
+    uint4 d0[kWsGroup], d1[kWsGroup];
+    // sched_barrier(0) keeps the loads in issue order (group 0 before group
+    // 1, chunk u before u+1): the in-order memory counter then lets each
+    // write wait for its own group only
 #pragma unroll
-    for (int u = 0; u < kWsInflight; ++u) data[u] = src(u);
-    for (int c0 = 0; c0 < nch; c0 += kWsInflight) {
+    for (int u = 0; u < kWsGroup; ++u) d0[u] = src(u);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kWsInflight; ++u) {
-            const int c = c0 + u;
-            if (c < nch) {
-                const int need = (c + 1) * kWsChunk - kWsRing;
-                while (lds_load_i(&ctl[1]) < need) {
-                    if (timed_out || wd.expired()) { timed_out = true; return; }
-                    __builtin_amdgcn_s_sleep(0);
-                }
-                lds_acquire();  // the solver's reads of these ring bytes have returned
-                ring[((c * kWsChunk) & (kWsRing - 1)) / 16 + lane] = data[u];
-                lds_release();
-                if (lane == 0) lds_store_i(&ctl[0], (c + 1) * kWsChunk);
-                data[u] = src(c + kWsInflight);
-            }
-        }
+    for (int u = 0; u < kWsGroup; ++u) d1[u] = src(kWsGroup + u);
+    __builtin_amdgcn_sched_barrier(0);
+    // No exit leaves the loop between the two groups' loads.
+    auto put = [&](int g, uint4 (&d)[kWsGroup]) {
+        // group g reuses the ring half of group g-2: wait until it is
+        // consumed (need <= the stream's bytes for every g <= ngroups)
+        const int need = (g + 1) * kGroupBytes - kWsRing;
+        while (lds_load_i(&ctl[1]) < need) __builtin_amdgcn_s_sleep(1);
+        lds_acquire();  // the solver's reads of that half have returned
+        uint4* half = ring + (g & 1) * (kGroupBytes / 16);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kWsGroup; ++u) half[u * (kWsChunk / 16) + lane] = d[u];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kWsGroup; ++u) d[u] = src((g + 2) * kWsGroup + u);
+        __builtin_amdgcn_sched_barrier(0);
+        lds_release();
+        if (lane == 0) lds_store_i(&ctl[0], min((g + 1) * kWsGroup, nch) * kWsChunk);
+    };
+    // both puts run on every iteration (a put past the last group rewrites a
+    // ring half the solver has consumed and does not move the loaded count):
+    // with no path that skips the second one, the compiler's count of the
+    // loads in flight at the loop head stays exact
+    for (int g = 0; g < ngroups; g += 2) {
+        put(g, d0);
+        put(g + 1, d1);
     }
 }
 
@@ -233,7 +351,7 @@ __device__ __forceinline__ void ws_import(const WsBlock& bk, const int2* __restr
                     for (int i = idx[k]; i < n; i += 64 * kWsImp) lds_store_u64(imp + i, kQuietNan);
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(2);
         }
     }
 }
@@ -250,7 +368,8 @@ __global__ __launch_bounds__(kWsThreads) void k_sweep_ws(const WsBlock* __restri
                                                         const int2* __restrict__ imports, const int* __restrict__ perm,
                                                         const double* b, double* out, double* reset, double* face,
                                                         double* face_rearm, const int* gate, int* ctr, int* err,
-                                                        uint64_t wd_limit, int max_rows, int max_imp) {
+                                                        uint64_t wd_limit, int max_rows, int max_imp, int n,
+                                                        int nface) {
     extern __shared__ double ws_lds[];
     __shared__ int ctl[4];  // bytes loaded, bytes consumed, records solved
     if (gate && *gate == 0) return;
@@ -288,12 +407,19 @@ __global__ __launch_bounds__(kWsThreads) void k_sweep_ws(const WsBlock* __restri
     if (face_rearm)
         for (int f = bk.r0 + tid; f < bk.r1; f += kWsThreads) face_rearm[f] = __longlong_as_double(kPending);
     __syncthreads();
-    bool timed_out = false;
+    bool timed_out = false, bad = false;
     const int wave = tid >> 6;
-    if (wave == 0) ws_solve<G, S, T, UPPER>(bk, ring, ctl, xl, out, face, wd, timed_out);
-    else if (wave == 1) ws_load(bk, stream, reinterpret_cast<uint4*>(ring), ctl, wd, timed_out);
+    if (wave == 0) {
+        // the solver is the critical path: the SIMD issues its instructions
+        // ahead of spinning loader and importer waves
+        __builtin_amdgcn_s_setprio(3);
+        ws_solve<G, S, T, UPPER>(bk, ring, ctl, xl, max_rows + max_imp, out, face, n, nface, wd, timed_out, bad);
+        __builtin_amdgcn_s_setprio(0);
+    }
+    else if (wave == 1) ws_load(bk, stream, reinterpret_cast<uint4*>(ring), ctl);
     else ws_import(bk, imports, imp, face, ctl, wd, timed_out);
     if (timed_out) atomicOr(err, kErrTimeout);
+    if (bad) atomicOr(err, kErrStream);
     retire_ticket(ctr);
 }
 
