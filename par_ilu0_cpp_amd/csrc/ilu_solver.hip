@@ -5,6 +5,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -763,9 +764,10 @@ void ensure_packs(Ext* e) {
 //     schedule order (so a block publishes and re-arms contiguous ranges);
 //   * each local level is cut into records of at most 64 / G rows; columns
 //     become LDS slots (in-block: schedule position within the block;
-//     imports: max_rows + import index; padding: the zero slot); values and
-//     diagonal are gathered into the stream after every factorization
-//     through (vdst, vsrc).
+//     imports: max_rows + import index; padding: the zero slot); the first
+//     S * G entries of each row are stored lane-major (S = 8 / G, at least
+//     1), the rest row-major; values and diagonal are gathered into the
+//     stream after every factorization through (vdst, vsrc).
 // ---------------------------------------------------------------------
 struct WsDir {
     std::vector<int> perm, blev, loff, kinv, blk_at, face_of;
@@ -837,8 +839,16 @@ inline int a16(int b) { return (b + 15) & ~15; }
 // Lays out and uploads one direction's stream; other_face[2 b], [2 b + 1]:
 // the other direction's face slots of block b, which this direction
 // re-arms.
+// Bytes of one record of cnt rows, w entries per row.
+int ws_record_bytes(int cnt, int w, int G, int sv, bool upper) {
+    const int S = std::max(1, 8 / G), L = cnt * G, wt = std::max(w - S * G, 0);
+    return 32 + a16(cnt * 8) + a16(L * S * 4) + a16(L * S * sv) + (wt > 0 ? a16(cnt * wt * 4) + a16(cnt * wt * sv) : 0) +
+           (upper ? a16(cnt * sv) : 0);
+}
+
 void ws_stream(Ext* e, bool upper, const WsDir& d, const std::vector<int>& other_face, int G, WsPack& wp) {
     const int n = e->n, nblk = e->nblk, P = 64 / G;
+    const int S = std::max(1, 8 / G);
     const int sv = e->precision == 32 ? 4 : 8;
     const std::vector<int>& rp = e->h_rp;
     const std::vector<int>& dg = e->h_dg;
@@ -862,11 +872,11 @@ void ws_stream(Ext* e, bool upper, const WsDir& d, const std::vector<int>& other
                     int w = 0;
                     for (int k = c0; k < c0 + cnt; ++k) w = std::max(w, entries(d.perm[k]));
                     w = (w + G - 1) / G * G;
-                    const int rb = 16 + a16(cnt * 8) + a16(cnt * w * 4) + a16(cnt * w * sv) + (upper ? a16(cnt * sv) : 0);
+                    const int rb = ws_record_bytes(cnt, w, G, sv, upper);
                     big = std::max(big, rb);
                     if ((off % kWsRing) + rb > kWsRing) off += kWsRing - off % kWsRing;
                     off += rb;
-                    nv += static_cast<long>(cnt) * w + (upper ? cnt : 0);
+                    nv += static_cast<long>(cnt) * (G * S + std::max(w - S * G, 0)) + (upper ? cnt : 0);
                 }
             bytes[p] = (off + kWsChunk - 1) / kWsChunk * kWsChunk;
             nval[p] = nv;
@@ -888,6 +898,7 @@ void ws_stream(Ext* e, bool upper, const WsDir& d, const std::vector<int>& other
     std::vector<int> vdst(static_cast<size_t>(std::max<long>(v0[nblk], 1))), vsrc(vdst.size(), -1);
     std::vector<int2> imports(static_cast<size_t>(std::max<long>(i0[nblk], 1)));
     std::vector<WsBlock> blocks(nblk);
+    std::atomic<bool> count_mismatch{false};
     // pass 2: fill
     parallel_for(nblk, [&](long lo, long hi) {
         std::vector<std::pair<int, int>> imap;
@@ -917,58 +928,93 @@ void ws_stream(Ext* e, bool upper, const WsDir& d, const std::vector<int>& other
             };
             char* base = reinterpret_cast<char*>(words.data()) + s0[p] * 16;
             const long sbyte0 = s0[p] * 16;
+            // the block's records: (first position, rows, width, bytes, offset)
+            struct Rec { int c0, cnt, w, rb; long off; };
+            std::vector<Rec> recs;
             long off = 0;
-            long vi = v0[p];
             for (int g = d.blev[p]; g < d.blev[p + 1]; ++g)
                 for (int c0 = d.loff[g]; c0 < d.loff[g + 1]; c0 += P) {
                     const int cnt = std::min(P, d.loff[g + 1] - c0);
                     int w = 0;
                     for (int k = c0; k < c0 + cnt; ++k) w = std::max(w, entries(d.perm[k]));
                     w = (w + G - 1) / G * G;
-                    const int rows_b = 16, cols_b = rows_b + a16(cnt * 8), vals_b = cols_b + a16(cnt * w * 4);
-                    const int diag_b = vals_b + a16(cnt * w * sv);
-                    const int rb = diag_b + (upper ? a16(cnt * sv) : 0);
+                    const int rb = ws_record_bytes(cnt, w, G, sv, upper);
                     if ((off % kWsRing) + rb > kWsRing) {
                         const int pad = static_cast<int>(kWsRing - off % kWsRing);
                         int* h = reinterpret_cast<int*>(base + off);
                         h[0] = 0; h[1] = 0; h[2] = pad; h[3] = 0;
                         off += pad;
                     }
-                    int* h = reinterpret_cast<int*>(base + off);
-                    h[0] = cnt;
-                    h[1] = w;
-                    h[2] = rb;
-                    h[3] = c0 - k0;
-                    int2* rows = reinterpret_cast<int2*>(base + off + rows_b);
-                    int* cols = reinterpret_cast<int*>(base + off + cols_b);
-                    for (int j = 0; j < cnt; ++j) {
-                        const int row = d.perm[c0 + j];
-                        rows[j] = make_int2(row, d.face_of[row]);
-                        const int e0 = upper ? dg[row] + 1 : rp[row];
-                        const int m = entries(row);
-                        for (int q = 0; q < w; ++q) {
-                            int slot = zero, src = -1;
-                            if (q < m) {
-                                const int c = ci[e0 + q];
-                                slot = bo[c] == blk ? d.kinv[c] - k0 : import_slot(c);
-                                src = e0 + q;
-                            }
-                            cols[j * w + q] = slot;
-                            vdst[vi] = static_cast<int>((sbyte0 + off + vals_b + static_cast<long>(j * w + q) * sv) / sv);
+                    recs.push_back({c0, cnt, w, rb, off});
+                    off += rb;
+                }
+            long vi = v0[p];
+            for (size_t r = 0; r < recs.size(); ++r) {
+                const Rec& rc = recs[r];
+                const int cnt = rc.cnt, w = rc.w, wt = std::max(w - S * G, 0), L = cnt * G;
+                const int rows_b = 32, lcols_b = rows_b + a16(cnt * 8), lvals_b = lcols_b + a16(L * S * 4);
+                const int tcols_b = lvals_b + a16(L * S * sv), tvals_b = tcols_b + (wt > 0 ? a16(cnt * wt * 4) : 0);
+                const int diag_b = tvals_b + (wt > 0 ? a16(cnt * wt * sv) : 0);
+                int* h = reinterpret_cast<int*>(base + rc.off);
+                h[0] = cnt;
+                h[1] = w;
+                h[2] = rc.rb;
+                h[3] = rc.c0 - k0;
+                if (r + 1 < recs.size()) {
+                    const Rec& nx = recs[r + 1];
+                    h[4] = nx.cnt;
+                    h[5] = nx.w;
+                    h[6] = nx.rb;
+                    h[7] = nx.c0 - k0;
+                }
+                int2* rows = reinterpret_cast<int2*>(base + rc.off + rows_b);
+                int* lcols = reinterpret_cast<int*>(base + rc.off + lcols_b);
+                int* tcols = reinterpret_cast<int*>(base + rc.off + tcols_b);
+                for (int j = 0; j < cnt; ++j) {
+                    const int row = d.perm[rc.c0 + j];
+                    rows[j] = make_int2(row, d.face_of[row]);
+                    const int e0 = upper ? dg[row] + 1 : rp[row];
+                    const int m = entries(row);
+                    // entry q of row j: (LDS slot, factor index or -1)
+                    auto entry = [&](int q, int& slot, int& src) {
+                        slot = zero;
+                        src = -1;
+                        if (q < m) {
+                            const int c = ci[e0 + q];
+                            slot = bo[c] == blk ? d.kinv[c] - k0 : import_slot(c);
+                            src = e0 + q;
+                        }
+                    };
+                    for (int t = 0; t < G; ++t)
+                        for (int u = 0; u < S; ++u) {
+                            const int li = (j * G + t) * S + u;
+                            int slot, src;
+                            entry(t + u * G, slot, src);
+                            lcols[li] = slot;
+                            vdst[vi] = static_cast<int>((sbyte0 + rc.off + lvals_b + static_cast<long>(li) * sv) / sv);
                             vsrc[vi] = src;
                             ++vi;
                         }
+                    for (int q = 0; q < wt; ++q) {
+                        int slot, src;
+                        entry(S * G + q, slot, src);
+                        tcols[j * wt + q] = slot;
+                        vdst[vi] = static_cast<int>((sbyte0 + rc.off + tvals_b + static_cast<long>(j * wt + q) * sv) / sv);
+                        vsrc[vi] = src;
+                        ++vi;
                     }
-                    if (upper)
-                        for (int j = 0; j < cnt; ++j) {
-                            vdst[vi] = static_cast<int>((sbyte0 + off + diag_b + static_cast<long>(j) * sv) / sv);
-                            vsrc[vi] = dg[d.perm[c0 + j]];
-                            ++vi;
-                        }
-                    off += rb;
                 }
+                if (upper)
+                    for (int j = 0; j < cnt; ++j) {
+                        vdst[vi] = static_cast<int>((sbyte0 + rc.off + diag_b + static_cast<long>(j) * sv) / sv);
+                        vsrc[vi] = dg[d.perm[rc.c0 + j]];
+                        ++vi;
+                    }
+            }
+            if (vi != v0[p + 1]) count_mismatch = true;
         }
     });
+    if (count_mismatch) throw std::logic_error("wave-specialised sweep: value count mismatch");
     wp.stream.resize(words.size() / 4);
     PILU_HIP(hipMemcpy(wp.stream.get(), words.data(), words.size() * 4, hipMemcpyHostToDevice));
     wp.vdst.resize(vdst.size());

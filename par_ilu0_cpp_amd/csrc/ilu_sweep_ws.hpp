@@ -14,9 +14,9 @@
 //     value-flags.
 //   * wave 1 streams the block's record stream -- headers, row ids,
 //     columns already translated to LDS slots, values, diagonal -- from HBM
-//     into the LDS ring in groups of eight 1 KiB chunks, two groups in
-//     flight, flow-controlled by two LDS counters (bytes loaded / bytes
-//     consumed).
+//     into a two-group LDS ring in groups of eight 1 KiB chunks, four
+//     groups in flight, flow-controlled by two LDS counters (bytes loaded /
+//     bytes consumed).
 //   * wave 2 imports: it polls the face buffer for the block's out-of-block
 //     dependencies, in order of first use and at most kWsLook records ahead
 //     of the solver, and writes each value into its LDS slot as a
@@ -28,11 +28,17 @@
 // an exact -0.0.
 //
 // Record layout (16-byte aligned sections): header int4 {rows, entries per
-// row w (a multiple of G), record bytes, first slot}; int2 {row, face slot
-// or -1} per row; int columns [rows][w] as LDS double indices; T values
-// [rows][w]; T diagonal [rows] (backward sweep).  A record never wraps the
-// ring (the stream is padded at ring boundaries) and is at most half the
-// ring.
+// row w (a multiple of G), record bytes, first slot}; the header of the
+// record that follows (zeros after the block's last record), so that its
+// body can be read without first reading its header; int2 {row, face slot
+// or -1} per row; the first S * G entries of every row lane-major -- lane
+// (j, t) of row j holds entries t, t+G, ..., t+(S-1)G as S int columns (LDS
+// double indices) and, in the next section, S values -- so each lane reads
+// its operands with a few wide, conflict-free LDS reads; entries past S * G
+// row-major (rows longer than S * G only), columns then values; T diagonal
+// [rows] (backward sweep).  Entries past a row's own count are (zero slot,
+// 0).  A record never wraps the ring (the stream is padded at ring
+// boundaries) and is at most half the ring.
 #pragma once
 
 #include "common.hpp"
@@ -42,7 +48,8 @@ namespace pilu {
 constexpr int kWsThreads = 192;   // solver, loader and importer waves
 constexpr int kWsChunk = 1024;    // bytes the loader moves per step (64 lanes x 16 B)
 constexpr int kWsRing = 16384;    // LDS ring of the record stream (bytes)
-constexpr int kWsGroup = 8;       // chunks the loader writes per ring-space check (two groups in flight)
+constexpr int kWsGroup = 8;       // chunks the loader writes per ring-space check (the ring holds two groups)
+constexpr int kWsFlight = 4;      // groups the loader keeps in flight (in registers) ahead of the ring
 constexpr int kWsImp = 4;         // imports each importer lane polls at once
 constexpr int kWsLook = 8;        // records ahead of the solver the importer polls
 constexpr int kErrStream = 1 << 9;  // error word: a record stream row or face id out of range
@@ -71,121 +78,136 @@ __device__ __forceinline__ void lds_store_u64(double* p, uint64_t v) {
 __device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
 __device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
 
-// Waits until the loader has put `need` bytes of the stream into the ring;
-// false on a watchdog expiry.
-__device__ __forceinline__ bool ws_wait_loaded(const int* ctl, int need, const Watchdog& wd) {
-    while (lds_load_i(&ctl[0]) < need) {
-        if (wd.expired()) return false;
-        __builtin_amdgcn_s_sleep(0);
-    }
-    lds_acquire();
-    return true;
-}
-
-// One record as the solver holds it: header, this lane's row data and the
-// columns and values of its first S entries (read from the ring one record
-// ahead of their use).
+// One record as the solver holds it: header, the next record's header,
+// this lane's row data and the columns and values of its first S entries.
 template <int S, typename T>
 struct WsRec {
-    int cnt, w, rbytes, slot0, off, vals_b;
+    int cnt, w, rbytes, slot0, off;
+    int4 nh;
     int2 rf;
     double dv, b;
     int c[S];
     T vr[S];
 };
 
-// Reads the next record at stream offset `off` (skipping ring padding);
-// false at the block's end or on a watchdog expiry (timed_out set).
-template <int G, int S, typename T, bool UPPER>
-__device__ __forceinline__ bool ws_fetch(int off, int bytes, const char* ring, const int* ctl, const double* xl,
-                                         int zero_slot, const Watchdog& wd, bool& timed_out, WsRec<S, T>& r) {
-    const int lane = threadIdx.x & 63;
-    const int j = lane / G, t = lane & (G - 1);
-    int cnt = 0, w = 0, rbytes = 0, slot0 = 0, base = 0;
-    while (true) {
-        if (off >= bytes) return false;
-        if (!ws_wait_loaded(ctl, off + 16, wd)) { timed_out = true; return false; }
-        base = off & (kWsRing - 1);
-        const int4 h = *reinterpret_cast<const int4*>(ring + base);
-        cnt = __builtin_amdgcn_readfirstlane(h.x);
-        w = __builtin_amdgcn_readfirstlane(h.y);
-        rbytes = __builtin_amdgcn_readfirstlane(h.z);
-        slot0 = __builtin_amdgcn_readfirstlane(h.w);
-        if (cnt > 0) break;
-        if (rbytes <= 0) return false;  // padding up to the block's end
-        off += rbytes;                   // padding up to a ring boundary
-    }
-    if (!ws_wait_loaded(ctl, off + rbytes, wd)) { timed_out = true; return false; }
-    const int rows_b = base + 16;
-    const int cols_b = rows_b + ws_align16(cnt * 8);
-    const int vals_b = cols_b + ws_align16(cnt * w * 4);
-    const int diag_b = vals_b + ws_align16(cnt * w * static_cast<int>(sizeof(T)));
-    const int jj = j < cnt ? j : 0;
-    r.cnt = cnt;
-    r.w = w;
-    r.rbytes = rbytes;
-    r.slot0 = slot0;
-    r.off = off;
-    r.vals_b = vals_b + jj * w * static_cast<int>(sizeof(T));
-    r.rf = reinterpret_cast<const int2*>(ring + rows_b)[jj];
-    r.dv = UPPER ? static_cast<double>(reinterpret_cast<const T*>(ring + diag_b)[jj]) : 1.0;
-    r.b = xl[slot0 + jj];
-    const int* cp = reinterpret_cast<const int*>(ring + cols_b) + jj * w;
-    const T* vp = reinterpret_cast<const T*>(ring + vals_b) + jj * w;
-    // branch-free: terms past the row's width read its last entry and are
-    // turned into (zero slot, 0), which adds -0.0
-    const int wl = w > 0 ? w - 1 : 0;
-#pragma unroll
-    for (int s = 0; s < S; ++s) r.c[s] = w > 0 ? cp[t + s * G < w ? t + s * G : wl] : zero_slot;
-#pragma unroll
-    for (int s = 0; s < S; ++s) r.vr[s] = w > 0 ? vp[t + s * G < w ? t + s * G : wl] : T(0);
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-        if (!(s * G < w)) {
-            r.c[s] = zero_slot;
-            r.vr[s] = T(0);
-        }
-    return true;
+__device__ __forceinline__ int4 ws_uniform(int4 v) {
+    return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                     __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
 }
 
-// Wave 0: walk the block's records, two stages deep: while record r's x
-// are read and summed, record r+1's header, row data, columns and values
-// are read from the ring, so only the x reads (which depend on the rows
-// just solved) stay on the critical path.
+// Issues the reads of a record's body whose header h is known (no wait:
+// the values are used later).
+template <int G, int S, typename T, bool UPPER>
+__device__ __forceinline__ void ws_body(int off, int4 h, const char* ring, const double* xl, WsRec<S, T>& r) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane / G, t = lane & (G - 1);
+    const int cnt = h.x, w = h.y;
+    const int base = off & (kWsRing - 1);
+    const int rows_b = base + 32;
+    const int lcols_b = rows_b + ws_align16(cnt * 8);
+    const int lvals_b = lcols_b + ws_align16(cnt * G * S * 4);
+    const int jj = j < cnt ? j : 0;
+    const int li = jj * G + t;
+    r.cnt = cnt;
+    r.w = w;
+    r.rbytes = h.z;
+    r.slot0 = h.w;
+    r.off = off;
+    r.nh = *reinterpret_cast<const int4*>(ring + base + 16);
+    r.rf = reinterpret_cast<const int2*>(ring + rows_b)[jj];
+    r.b = xl[h.w + jj];
+    const int* cp = reinterpret_cast<const int*>(ring + lcols_b) + li * S;
+    const T* vp = reinterpret_cast<const T*>(ring + lvals_b) + li * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.c[s] = cp[s];
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.vr[s] = vp[s];
+    if (UPPER) {
+        const int wt = w > S * G ? w - S * G : 0;
+        const int tcols_b = lvals_b + ws_align16(cnt * G * S * static_cast<int>(sizeof(T)));
+        const int diag_b = tcols_b + (wt > 0 ? ws_align16(cnt * wt * 4) + ws_align16(cnt * wt * static_cast<int>(sizeof(T))) : 0);
+        r.dv = static_cast<double>(reinterpret_cast<const T*>(ring + diag_b)[jj]);
+    }
+    else r.dv = 1.0;
+}
+
+// Wave 0: walk the block's records, two stages deep: record r+1's body
+// (its header is in record r) is read from the ring together with record
+// r's x, so each record costs one LDS round trip, the x reads (which
+// depend on the rows just solved).  Record r is never held back waiting
+// for r+1's bytes: if the loader has not put them in the ring yet, r+1 is
+// read after r is solved.  The loaded count is re-read only when a record
+// ends past the last count seen.
 //
 // Ring flow control: the solver holds at most one record's ring bytes
 // unconsumed while it waits for the loader.  A record whose entries all sit
-// in registers after ws_fetch (w <= S * G) is released before record r+1
-// is fetched; a longer row's record is read again by the tail loop, so it
-// is released after its rows are solved and record r+1 is fetched then.
-// Holding r while waiting for r+1 could deadlock: the loader refills a
-// ring half only once the half before it is consumed, and two records can
-// together exceed the free half.
+// in registers after ws_body (w <= S * G) is released before record r+1 is
+// read; a longer row's record is read again by the tail loop, so it is
+// released after its rows are solved and record r+1 is read then.  Holding
+// r while waiting for r+1 could deadlock: the loader refills a ring half
+// only once the half before it is consumed, and two records can together
+// exceed the free half.
 template <int G, int S, typename T, bool UPPER>
-__device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* xl, int zero_slot, double* out,
-                         double* face, int n, int nface, const Watchdog& wd, bool& timed_out, bool& bad) {
+__device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* xl, double* out, double* face, int n,
+                         int nface, const Watchdog& wd, bool& timed_out, bool& bad) {
     const int lane = threadIdx.x & 63;
     const int j = lane / G;
     const int t = lane & (G - 1);
     const int bytes = (bk.s1 - bk.s0) * 16;
+    int loaded = 0;  // last loaded count seen
+    // waits until `need` bytes are in the ring; false on a watchdog expiry
+    auto wait_loaded = [&](int need) -> bool {
+        while (loaded < need) {
+            loaded = __builtin_amdgcn_readfirstlane(lds_load_i(&ctl[0]));
+            if (loaded >= need) break;
+            if (wd.expired()) return false;
+            __builtin_amdgcn_s_sleep(0);
+        }
+        lds_acquire();
+        return true;
+    };
     WsRec<S, T> cur, nxt;
-    bool have = ws_fetch<G, S, T, UPPER>(0, bytes, ring, ctl, xl, zero_slot, wd, timed_out, cur);
+    bool live = bytes > 0;
+    if (live) {
+        if (!wait_loaded(16)) { timed_out = true; live = false; }
+    }
+    if (live) {
+        const int4 h = ws_uniform(*reinterpret_cast<const int4*>(ring));
+        live = h.x > 0;
+        if (live && !wait_loaded(h.z)) { timed_out = true; live = false; }
+        if (live) ws_body<G, S, T, UPPER>(0, h, ring, xl, cur);
+    }
     int rec = 0;
-    while (have) {
+    while (live) {
+        const int4 nh = ws_uniform(cur.nh);
+        const bool has_next = nh.x > 0;
+        const int end = cur.off + cur.rbytes;
+        // the next record starts at the next ring boundary if it would wrap
+        const int nxt_off = has_next && (end & (kWsRing - 1)) + nh.z > kWsRing ? (end + kWsRing - 1) & ~(kWsRing - 1)
+                                                                                : end;
+        const int nxt_end = nxt_off + nh.z;
         const bool act = j < cur.cnt;
         const bool tail = cur.w > S * G;
-        const int next_off = cur.off + cur.rbytes;
         if (!tail) {
             lds_release();  // this record's ring reads have returned
-            if (lane == 0) lds_store_i(&ctl[1], next_off);
+            if (lane == 0) lds_store_i(&ctl[1], end);
         }
         // x of this record's first S entries: every read at once
         uint64_t xb[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) xb[s] = lds_load_u64(xl + cur.c[s]);
-        // the next record, while those reads are in flight
-        bool more = false;
-        if (!tail) more = ws_fetch<G, S, T, UPPER>(next_off, bytes, ring, ctl, xl, zero_slot, wd, timed_out, nxt);
+        // the next record's body, while those reads are in flight, if loaded
+        bool fetched = false;
+        if (has_next && !tail) {
+            if (loaded < nxt_end) {
+                loaded = __builtin_amdgcn_readfirstlane(lds_load_i(&ctl[0]));
+                lds_acquire();
+            }
+            if (loaded >= nxt_end) {
+                ws_body<G, S, T, UPPER>(nxt_off, nh, ring, xl, nxt);
+                fetched = true;
+            }
+        }
         bool pend = false;
 #pragma unroll
         for (int s = 0; s < S; ++s) pend |= xb[s] == kPending;
@@ -204,10 +226,13 @@ __device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* 
         for (int s = 0; s < S; ++s) acc = fma(-__longlong_as_double(xb[s]), static_cast<double>(cur.vr[s]), acc);
         if (act && tail) {
             // entries past the first S per lane (rows longer than S * G)
-            const int rows_b = (cur.off & (kWsRing - 1)) + 16;
-            const int* cp = reinterpret_cast<const int*>(ring + rows_b + ws_align16(cur.cnt * 8)) + j * cur.w;
-            const T* vp = reinterpret_cast<const T*>(ring + cur.vals_b);
-            for (int q = t + S * G; q < cur.w; q += G) {
+            const int wt = cur.w - S * G;
+            const int base = cur.off & (kWsRing - 1);
+            const int tcols_b = base + 32 + ws_align16(cur.cnt * 8) + ws_align16(cur.cnt * G * S * 4) +
+                                ws_align16(cur.cnt * G * S * static_cast<int>(sizeof(T)));
+            const int* cp = reinterpret_cast<const int*>(ring + tcols_b) + j * wt;
+            const T* vp = reinterpret_cast<const T*>(ring + tcols_b + ws_align16(cur.cnt * wt * 4)) + j * wt;
+            for (int q = t; q < wt; q += G) {
                 const int cq = cp[q];
                 uint64_t bits = lds_load_u64(xl + cq);
                 while (bits == kPending) {
@@ -235,12 +260,15 @@ __device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* 
         ++rec;
         if (tail) {
             lds_release();  // the tail loop's ring reads have returned
-            if (lane == 0) lds_store_i(&ctl[1], next_off);
+            if (lane == 0) lds_store_i(&ctl[1], end);
         }
         if (lane == 0) lds_store_i(&ctl[2], rec);
-        if (tail) more = ws_fetch<G, S, T, UPPER>(next_off, bytes, ring, ctl, xl, zero_slot, wd, timed_out, nxt);
+        if (!has_next) break;
+        if (!fetched) {
+            if (!wait_loaded(nxt_end)) { timed_out = true; break; }
+            ws_body<G, S, T, UPPER>(nxt_off, nh, ring, xl, nxt);
+        }
         cur = nxt;
-        have = more;
     }
     if (lane == 0) {
         lds_store_i(&ctl[1], bytes);
@@ -249,10 +277,12 @@ __device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* 
 }
 
 // Wave 1: stream the block's records into the ring.  The ring holds two
-// groups of kWsGroup chunks; the loads of group g+2 are issued as group g
-// is written, so two groups are always in flight.  Inside a group there is
-// no control flow, which lets the compiler wait for each chunk with an
-// exact count of the younger loads instead of draining them all.
+// groups of kWsGroup chunks; kWsFlight groups are in flight in registers
+// ahead of it (the loads of group g+kWsFlight are issued as group g is
+// written), so a block streams kWsFlight * 8 KiB per memory latency.
+// Inside a group there is no control flow, which lets the compiler wait
+// for each chunk with an exact count of the younger loads instead of
+// draining them all.
 //
 // The loader has no watchdog of its own and never writes a ring half
 // before the solver has consumed it: its only wait is for the consumed
@@ -261,6 +291,7 @@ __device__ void ws_solve(const WsBlock& bk, const char* ring, int* ctl, double* 
 __device__ __forceinline__ void ws_load(const WsBlock& bk, const uint4* __restrict__ stream, uint4* ring, int* ctl) {
     constexpr int kGroupBytes = kWsGroup * kWsChunk;
     static_assert(2 * kGroupBytes == kWsRing, "the ring holds two groups");
+    static_assert(kWsFlight % 2 == 0, "groups in flight alternate over the two ring halves");
     const int lane = threadIdx.x & 63;
     const int nch = (bk.s1 - bk.s0) / (kWsChunk / 16);
     if (nch <= 0) return;
@@ -270,41 +301,42 @@ __device__ __forceinline__ void ws_load(const WsBlock& bk, const uint4* __restri
         const int cc = c < nch ? c : nch - 1;
         return stream[src0 + static_cast<long>(cc) * (kWsChunk / 16) + lane];
     };
-    uint4 d0[kWsGroup], d1[kWsGroup];
-    // sched_barrier(0) keeps the loads in issue order (group 0 before group
-    // 1, chunk u before u+1): the in-order memory counter then lets each
-    // write wait for its own group only
+    uint4 d[kWsFlight][kWsGroup];
+    // sched_barrier(0) keeps the loads in issue order (group before group,
+    // chunk u before u+1): the in-order memory counter then lets each write
+    // wait for its own group only
 #pragma unroll
-    for (int u = 0; u < kWsGroup; ++u) d0[u] = src(u);
-    __builtin_amdgcn_sched_barrier(0);
+    for (int q = 0; q < kWsFlight; ++q) {
 #pragma unroll
-    for (int u = 0; u < kWsGroup; ++u) d1[u] = src(kWsGroup + u);
-    __builtin_amdgcn_sched_barrier(0);
-    // No exit leaves the loop between the two groups' loads.
-    auto put = [&](int g, uint4 (&d)[kWsGroup]) {
+        for (int u = 0; u < kWsGroup; ++u) d[q][u] = src(q * kWsGroup + u);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // No exit leaves the loop between the groups' loads.
+    auto put = [&](int g, uint4 (&dq)[kWsGroup]) {
         // group g reuses the ring half of group g-2: wait until it is
-        // consumed (need <= the stream's bytes for every g <= ngroups)
-        const int need = (g + 1) * kGroupBytes - kWsRing;
+        // consumed.  Puts past the last group wait for the whole stream
+        // (the solver's final count), never for more.
+        const int need = min((g + 1) * kGroupBytes - kWsRing, nch * kWsChunk);
         while (lds_load_i(&ctl[1]) < need) __builtin_amdgcn_s_sleep(1);
         lds_acquire();  // the solver's reads of that half have returned
         uint4* half = ring + (g & 1) * (kGroupBytes / 16);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kWsGroup; ++u) half[u * (kWsChunk / 16) + lane] = d[u];
+        for (int u = 0; u < kWsGroup; ++u) half[u * (kWsChunk / 16) + lane] = dq[u];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kWsGroup; ++u) d[u] = src((g + 2) * kWsGroup + u);
+        for (int u = 0; u < kWsGroup; ++u) dq[u] = src((g + kWsFlight) * kWsGroup + u);
         __builtin_amdgcn_sched_barrier(0);
         lds_release();
         if (lane == 0) lds_store_i(&ctl[0], min((g + 1) * kWsGroup, nch) * kWsChunk);
     };
-    // both puts run on every iteration (a put past the last group rewrites a
-    // ring half the solver has consumed and does not move the loaded count):
-    // with no path that skips the second one, the compiler's count of the
+    // every put of an iteration runs (a put past the last group rewrites a
+    // ring half the solver has consumed and does not move the loaded
+    // count): with no path that skips one, the compiler's count of the
     // loads in flight at the loop head stays exact
-    for (int g = 0; g < ngroups; g += 2) {
-        put(g, d0);
-        put(g + 1, d1);
+    for (int g = 0; g < ngroups; g += kWsFlight) {
+#pragma unroll
+        for (int q = 0; q < kWsFlight; ++q) put(g + q, d[q]);
     }
 }
 
@@ -413,7 +445,7 @@ __global__ __launch_bounds__(kWsThreads) void k_sweep_ws(const WsBlock* __restri
         // the solver is the critical path: the SIMD issues its instructions
         // ahead of spinning loader and importer waves
         __builtin_amdgcn_s_setprio(3);
-        ws_solve<G, S, T, UPPER>(bk, ring, ctl, xl, max_rows + max_imp, out, face, n, nface, wd, timed_out, bad);
+        ws_solve<G, S, T, UPPER>(bk, ring, ctl, xl, out, face, n, nface, wd, timed_out, bad);
         __builtin_amdgcn_s_setprio(0);
     }
     else if (wave == 1) ws_load(bk, stream, reinterpret_cast<uint4*>(ring), ctl);
