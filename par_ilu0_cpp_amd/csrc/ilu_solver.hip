@@ -1038,9 +1038,11 @@ void ws_stream(Ext* e, bool upper, const WsDir& d, const std::vector<int>& other
     (void)n;
 }
 
-// LDS of k_sweep_ws for a direction: block x, imports, zero slot, ring.
+// LDS of k_sweep_ws for a direction: block x, imports, zero slot, import
+// list, ring.
 size_t ws_lds_bytes(const Ext* e, const WsPack& wp) {
-    return sizeof(double) * static_cast<size_t>((e->max_block_rows + wp.max_imp + 2) & ~1) + kWsRing;
+    return sizeof(double) * static_cast<size_t>((e->max_block_rows + wp.max_imp + 2) & ~1) +
+           static_cast<size_t>(a16(wp.max_imp * 8)) + kWsRing;
 }
 
 void gather_ws(Ext* e, WsPack& wp) {
@@ -1933,7 +1935,20 @@ std::string IluSolver::DescribeSchedule() const {
     const bool live_u = e->nblk > 0 && e->pack_u.version == e->part_version;
     js += "\"lv_cached_levels\": [" + std::to_string(e->lvcached_l) + ", " + std::to_string(e->lvcached_u) + "], ";
     js += "\"pack_l\": " + (live_l ? pack(e->pack_l) : std::string("null"));
-    js += ", \"pack_u\": " + (live_u ? pack(e->pack_u) : std::string("null")) + "}";
+    js += ", \"pack_u\": " + (live_u ? pack(e->pack_u) : std::string("null"));
+    // wave-specialised streams: LDS per workgroup and how many fit a CU's 160 KiB
+    auto ws = [&](const pilu::WsPack& wp) {
+        if (wp.version != e->part_version || !wp.stream.get()) return std::string("null");
+        const size_t lds = pilu::ws_lds_bytes(e, wp);
+        char b[256];
+        std::snprintf(b, sizeof(b),
+                      "{\"lanes\": %d, \"stream_bytes\": %ld, \"max_imports\": %d, \"face_slots\": %d, \"lds_bytes\": %zu, "
+                      "\"workgroups_per_cu\": %zu}",
+                      wp.g, static_cast<long>(wp.nstream) * 16, wp.max_imp, wp.nface, lds,
+                      static_cast<size_t>(160 * 1024) / std::max<size_t>(lds, 1));
+        return std::string(b);
+    };
+    js += ", \"ws_l\": " + ws(e->ws_l) + ", \"ws_u\": " + ws(e->ws_u) + "}";
     return js;
 }
 

@@ -341,7 +341,9 @@ __device__ __forceinline__ void ws_load(const WsBlock& bk, const uint4* __restri
 }
 
 // Wave 2: import the block's out-of-block dependencies into their slots.
-__device__ __forceinline__ void ws_import(const WsBlock& bk, const int2* __restrict__ imports, double* imp,
+// imports: the block's (face slot, first record) list, staged in LDS, so a
+// lane moves on to its next import without a memory round trip.
+__device__ __forceinline__ void ws_import(const WsBlock& bk, const int2* imports, double* imp,
                                           const double* face, const int* ctl, const Watchdog& wd,
                                           bool& timed_out) {
     const int lane = threadIdx.x & 63;
@@ -351,7 +353,7 @@ __device__ __forceinline__ void ws_import(const WsBlock& bk, const int2* __restr
 #pragma unroll
     for (int k = 0; k < kWsImp; ++k) {
         idx[k] = lane + 64 * k;
-        ent[k] = idx[k] < n ? imports[bk.i0 + idx[k]] : make_int2(0, 0);
+        ent[k] = idx[k] < n ? imports[idx[k]] : make_int2(0, 0);
     }
     while (true) {
         const int rec = lds_load_i(&ctl[2]);
@@ -368,7 +370,7 @@ __device__ __forceinline__ void ws_import(const WsBlock& bk, const int2* __restr
             if (want[k] && bits[k] != kPending) {
                 lds_store_u64(imp + idx[k], bits[k]);
                 idx[k] += 64 * kWsImp;
-                ent[k] = idx[k] < n ? imports[bk.i0 + idx[k]] : make_int2(0, 0);
+                ent[k] = idx[k] < n ? imports[idx[k]] : make_int2(0, 0);
                 progress = true;
             }
             left |= idx[k] < n;
@@ -390,7 +392,8 @@ __device__ __forceinline__ void ws_import(const WsBlock& bk, const int2* __restr
 
 // One workgroup per block, blocks taken in processing order by ticket.
 // LDS (dynamic): the block's x (max_rows), the import slots (max_imp), one
-// slot holding 0.0 (padding entries), then the ring.  reset (optional):
+// slot holding 0.0 (padding entries), the import list (max_imp int2), then
+// the ring.  reset (optional):
 // the other sweep's output, re-armed (kPending) row by row after b is
 // staged; face_rearm: the other direction's face buffer, whose slots
 // [r0, r1) this block re-arms.
@@ -410,7 +413,8 @@ __global__ __launch_bounds__(kWsThreads) void k_sweep_ws(const WsBlock* __restri
     const Watchdog wd(wd_limit);
     double* xl = ws_lds;
     double* imp = ws_lds + max_rows;
-    char* ring = reinterpret_cast<char*>(ws_lds + ((max_rows + max_imp + 1 + 1) & ~1));
+    int2* imp_ent = reinterpret_cast<int2*>(ws_lds + ((max_rows + max_imp + 1 + 1) & ~1));
+    char* ring = reinterpret_cast<char*>(imp_ent) + ws_align16(max_imp * 8);
     const int tid = threadIdx.x;
     if (tid < 4) ctl[tid] = 0;
     // stage b (eight rows per thread in flight), arm the import slots, the
@@ -434,7 +438,10 @@ __global__ __launch_bounds__(kWsThreads) void k_sweep_ws(const WsBlock* __restri
             }
         }
     }
-    for (int i = tid; i < bk.i1 - bk.i0; i += kWsThreads) imp[i] = __longlong_as_double(kPending);
+    for (int i = tid; i < bk.i1 - bk.i0; i += kWsThreads) {
+        imp[i] = __longlong_as_double(kPending);
+        imp_ent[i] = imports[bk.i0 + i];
+    }
     if (tid == 0) imp[max_imp] = 0.0;
     if (face_rearm)
         for (int f = bk.r0 + tid; f < bk.r1; f += kWsThreads) face_rearm[f] = __longlong_as_double(kPending);
@@ -449,7 +456,7 @@ __global__ __launch_bounds__(kWsThreads) void k_sweep_ws(const WsBlock* __restri
         __builtin_amdgcn_s_setprio(0);
     }
     else if (wave == 1) ws_load(bk, stream, reinterpret_cast<uint4*>(ring), ctl);
-    else ws_import(bk, imports, imp, face, ctl, wd, timed_out);
+    else ws_import(bk, imp_ent, imp, face, ctl, wd, timed_out);
     if (timed_out) atomicOr(err, kErrTimeout);
     if (bad) atomicOr(err, kErrStream);
     retire_ticket(ctr);
