@@ -68,9 +68,15 @@ void ReadCsrMatrixMM1(CsrMatrix* mat, const char* filename);
 #define CsrMatVec LLJBASH_DECL(CsrMatVec)
 void CsrMatVec(const CsrMatrix* mat, const double* vec, double* sol);
 
-/* Fill-reducing orderings.  The reference delegates to SuiteSparse AMD and
- * CHOLMOD nested dissection (src/csr_matrix.cpp:173-225); this build ships
- * neither library, so both return -1 and leave p/ip untouched.
+/* Fill- and depth-reducing symmetric orderings of the pattern of A + A^T:
+ * p[k] = original index of the k-th row of the permuted matrix, and, if ip
+ * is not null, ip[p[k]] = k; apply as B[ip[i]][ip[j]] = A[i][j].  The
+ * reference delegates to SuiteSparse (src/csr_matrix.cpp:173-225); this
+ * build has its own approximate minimum degree (CsrAmdOrder) and nested
+ * dissection by recursive level-structure bisection (CsrNdOrder), so the
+ * permutations are not SuiteSparse's.  Return values keep the reference's
+ * conventions: CsrAmdOrder 0 on success and -1 on failure, CsrNdOrder 1 on
+ * success and 0 on failure.
  * reference: include/csr_matrix.h:45-49 */
 #define CsrAmdOrder LLJBASH_DECL(CsrAmdOrder)
 int CsrAmdOrder(const CsrMatrix* mat, int* p, int* ip);

@@ -28,6 +28,9 @@ __all__ = [
     "block_diagonal",
     "IlukSymbolic",
     "iluk_symbolic_levels",
+    "CsrAmdOrder",
+    "CsrNdOrder",
+    "permute_symmetric",
     "device_available",
     "lib",
 ]
@@ -115,6 +118,46 @@ def IlukSymbolic(a: HostCsr, lof: int, f: CsrMatrix, nzmap: np.ndarray | None = 
     """include/iluk.h (reference include/iluk.h:15): returns nnz(F)."""
     p = _ptr(nzmap, C.c_int) if nzmap is not None else None
     return int(fn("IlukSymbolic")(C.byref(a.mat), int(lof), C.byref(f), p))
+
+
+def _order(name: str, a: HostCsr):
+    n = a.size
+    p = np.zeros(max(n, 1), dtype=np.int32)
+    ip = np.zeros(max(n, 1), dtype=np.int32)
+    rc = int(fn(name)(C.byref(a.mat), _ptr(p, C.c_int), _ptr(ip, C.c_int)))
+    return rc, p[:n].copy(), ip[:n].copy()
+
+
+def CsrAmdOrder(a: HostCsr):
+    """include/csr_matrix.h CsrAmdOrder (reference include/csr_matrix.h:47,
+    src/csr_matrix.cpp:173-195): approximate minimum degree ordering of the
+    pattern of A + A^T, on the host.  Returns (rc, p, ip) with p[k] the old
+    index of new row k and ip[p[k]] = k; rc is 0 on success, -1 on failure."""
+    return _order("CsrAmdOrder", a)
+
+
+def CsrNdOrder(a: HostCsr):
+    """include/csr_matrix.h CsrNdOrder (reference include/csr_matrix.h:48,
+    src/csr_matrix.cpp:197-225): nested dissection ordering of the pattern of
+    A + A^T, on the host.  Returns (rc, p, ip); rc is 1 on success, 0 on
+    failure (the reference returns CHOLMOD's status test)."""
+    return _order("CsrNdOrder", a)
+
+
+def permute_symmetric(row_ptr, col_idx, value, ip):
+    """B = P A P^T with B[ip[i], ip[j]] = A[i, j], columns sorted in each row:
+    the permutation the reference benchmark applies (test/benchmark.cpp:44-80)."""
+    rp = np.asarray(row_ptr, dtype=np.int64)
+    ci = np.asarray(col_idx, dtype=np.int64)
+    ip = np.asarray(ip, dtype=np.int64)
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    nr, nc = ip[rows], ip[ci]
+    order = np.lexsort((nc, nr))
+    out_rp = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(out_rp, nr + 1, 1)
+    return (np.cumsum(out_rp).astype(np.int32), nc[order].astype(np.int32),
+            np.asarray(value, dtype=np.float64)[order])
 
 
 def iluk_symbolic_levels(a: HostCsr, lof: int):

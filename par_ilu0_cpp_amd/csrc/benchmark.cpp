@@ -146,9 +146,12 @@ int main(int argc, char** argv) {
         }
         if (opt.amd || opt.nd) {
             std::vector<int> p(n), ip(n);
-            const int rc = opt.amd ? CsrAmdOrder(&csr, p.data(), ip.data()) : CsrNdOrder(&csr, p.data(), ip.data());
-            if (rc != 0) {
-                std::fprintf(stderr, "error: %s ordering is not available in this build\n", opt.amd ? "AMD" : "ND");
+            // CsrAmdOrder returns 0 on success, CsrNdOrder 1 (the reference's
+            // conventions; its benchmark ignores both)
+            const bool ok = opt.amd ? CsrAmdOrder(&csr, p.data(), ip.data()) == 0
+                                    : CsrNdOrder(&csr, p.data(), ip.data()) == 1;
+            if (!ok) {
+                std::fprintf(stderr, "error: %s ordering failed\n", opt.amd ? "AMD" : "ND");
                 return 1;
             }
             permute(&csr, ip);

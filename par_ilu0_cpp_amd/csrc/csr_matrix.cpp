@@ -182,8 +182,4 @@ void CsrMatVec(const CsrMatrix* mat, const double* vec, double* sol) {
     }
 }
 
-int CsrAmdOrder(const CsrMatrix*, int*, int*) { return -1; }
-
-int CsrNdOrder(const CsrMatrix*, int*, int*) { return -1; }
-
 }  // extern "C"

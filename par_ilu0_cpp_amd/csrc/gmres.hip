@@ -8,6 +8,7 @@
 // block in device memory, and every vector kernel of a step is gated by the
 // control block's step flag.  The host only looks at the control block one
 // step behind (to stop enqueuing) and at the end of each cycle.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -471,25 +472,37 @@ MgsPlan plan_mgs(int n) {
 }
 
 template <int R>
-void launch_mgs_r(const MgsPlan& p, hipStream_t s, int n, int i, int m, const double* vv, double* w,
-                  const double* part0, double* pbuf, double* col, double* tnorm, const pilu::GmresCtl* ctl, int* err,
-                  uint64_t wd) {
+hipError_t launch_mgs_r(const MgsPlan& p, hipStream_t s, int n, int i, int m, const double* vv, double* w,
+                        const double* part0, double* pbuf, double* col, double* tnorm, const pilu::GmresCtl* ctl,
+                        int* err, uint64_t wd) {
     int chunk = p.chunk;
     void* args[] = {&n, &i, &m, &chunk, &vv, &w, &part0, &pbuf, &col, &tnorm, &ctl, &err, &wd};
-    PILU_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pilu::k_mgs_all<R>), dim3(p.nwg),
-                                        dim3(pilu::kMgsThreads), args, 0, s));
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pilu::k_mgs_all<R>), dim3(p.nwg),
+                                      dim3(pilu::kMgsThreads), args, 0, s);
 }
 
-void launch_mgs_all(const MgsPlan& p, hipStream_t s, int n, int i, int m, const double* vv, double* w,
+// Set once the runtime refuses a cooperative launch in this process (it
+// does, for instance, while another process on the GPU holds the
+// cooperative queue); solves then use the per-term kernels.
+std::atomic<bool> g_coop_refused{false};
+
+// The one-launch Gram-Schmidt; false (error cleared) when the runtime
+// refuses the cooperative launch, so the caller can use the per-term kernels.
+bool launch_mgs_all(const MgsPlan& p, hipStream_t s, int n, int i, int m, const double* vv, double* w,
                     const double* part0, double* pbuf, double* col, double* tnorm, const pilu::GmresCtl* ctl,
                     int* err, uint64_t wd) {
+    hipError_t rc;
     switch (p.r) {
-        case 1: launch_mgs_r<1>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
-        case 2: launch_mgs_r<2>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
-        case 4: launch_mgs_r<4>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
-        case 8: launch_mgs_r<8>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
-        default: launch_mgs_r<16>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        case 1: rc = launch_mgs_r<1>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        case 2: rc = launch_mgs_r<2>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        case 4: rc = launch_mgs_r<4>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        case 8: rc = launch_mgs_r<8>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
+        default: rc = launch_mgs_r<16>(p, s, n, i, m, vv, w, part0, pbuf, col, tnorm, ctl, err, wd); break;
     }
+    if (rc == hipSuccess) return true;
+    (void)hipGetLastError();
+    g_coop_refused = true;
+    return false;
 }
 
 struct TimedPair {
@@ -531,7 +544,7 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const std::function<voi
     const int lanes = pilu::lanes_for_avg(avg);
     // One-launch Gram-Schmidt when each of the co-resident workgroups can
     // keep its rows of w in registers.
-    const MgsPlan mgs = plan_mgs(n);
+    const MgsPlan mgs = g_coop_refused ? MgsPlan{} : plan_mgs(n);
     double* pbuf = nullptr;
     if (mgs.nwg > 0) {
         e->pbuf.reserve(static_cast<size_t>(2) * (m + 1) * mgs.nwg);
@@ -632,9 +645,12 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const std::function<voi
             t_begin(0);
             spmv(lanes, s, n, rp, ci, val, 1.0, zi, 0.0, nullptr, vi1, vv, false, part0, step_gate);
             t_end();
+            // a refused cooperative launch switches this and later steps to
+            // the per-term kernels
+            if (pbuf && !launch_mgs_all(mgs, s, n, i, m, vv, vi1, part0, pbuf, hh + static_cast<long>(i) * im1,
+                                        tnorm, dctl, mgs_err, wd))
+                pbuf = nullptr;
             if (pbuf) {
-                launch_mgs_all(mgs, s, n, i, m, vv, vi1, part0, pbuf, hh + static_cast<long>(i) * im1, tnorm, dctl,
-                               mgs_err, wd);
                 pilu::k_arnoldi_finish<<<1, pilu::kVecThreads, 0, s>>>(dctl, nullptr, tnorm, hh, e->hist.get());
             } else {
                 double* pin = part0;
@@ -672,7 +688,7 @@ std::pair<bool, int> solve_core(GExt* e, IluSolver* ilu, const std::function<voi
     pilu::k_sqrt_sum<<<1, pilu::kVecThreads, 0, s>>>(part0, part1);
     PILU_HIP(hipMemcpyAsync(&e->true_res, part1, sizeof(double), hipMemcpyDeviceToHost, s));
     if (ie) pilu::check_sweep_errors(ie);
-    if (pbuf) {
+    if (mgs.nwg > 0) {  // cooperative launches may have run (before a refusal too)
         int werr = 0;
         PILU_HIP(hipMemcpyAsync(&werr, mgs_err, sizeof(int), hipMemcpyDeviceToHost, s));
         PILU_HIP(hipStreamSynchronize(s));

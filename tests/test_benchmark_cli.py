@@ -52,11 +52,30 @@ def test_missing_matrix_is_an_error():
     assert r.returncode == 1 and "--mat" in r.stderr
 
 
-def test_unavailable_ordering_is_reported(tmp_path):
+@pytest.mark.gpu
+@pytest.mark.parametrize("flag", ["--amd", "--nd"])
+def test_ordered_run_matches_oracle(oracle, tmp_path, flag):
+    """--amd / --nd permute A symmetrically before ILU(k) (reference
+    test/benchmark.cpp:44-80); the oracle solves the same permuted system."""
+    import par_ilu0_cpp_amd as P
+    a = gen.convdiff3d(14)
     p = tmp_path / "a.mtx"
-    write_mm(p, gen.poisson2d(6))
-    r = run("-m", p, "--amd")
-    assert r.returncode == 1 and "AMD" in r.stderr
+    write_mm(p, a)
+    r = run("-m", p, "-k", 1, "-r", 30, "-i", 500, "-n", 2, "-w", flag)
+    assert r.returncode == 0, r.stderr
+    its = [int(v) for v in re.findall(r"^iter: (\d+)$", r.stdout, re.M)]
+    assert len(its) == 3 and len(set(its)) == 1
+    order = P.CsrAmdOrder if flag == "--amd" else P.CsrNdOrder
+    rc, perm, ip = order(P.HostCsr(a.row_ptr, a.col_idx, a.value))
+    assert rc == (0 if flag == "--amd" else 1)
+    brp, bci, bva = P.permute_symmetric(a.row_ptr, a.col_idx, a.value, ip)
+    b = oracle.spmv(brp, bci, bva, np.ones(a.n))
+    o = oracle.iluk_symbolic(brp, bci, bva, 1)
+    fac, dg, _ = oracle.ilu_factorize(o[0], o[1], o[3])
+    ok_o, its_o, _, _ = oracle.gmres((brp, bci, bva), b, 1e-8, 30, 500, factors=(o[0], o[1], dg, fac))
+    assert ok_o and abs(its[0] - its_o) <= 1
+    assert field(r.stdout, "knnz") == o[0][-1]
+    assert field(r.stdout, "max_err") < 1e-6
 
 
 @pytest.mark.gpu
