@@ -124,13 +124,20 @@ def test_read_mm_nnz_mismatch_exits(tmp_path):
     assert r.returncode == 255 and "nnz number error" in r.stdout
 
 
-def test_orderings_unavailable():
+def test_ordering_return_conventions():
+    """The reference's codes (src/csr_matrix.cpp:194, :216-224): CsrAmdOrder
+    0 on success and -1 on failure, CsrNdOrder 1 on success and 0 on
+    failure; ip may be null."""
     a = P.HostCsr.from_csr(gen.poisson2d(4))
     p = np.zeros(16, dtype=np.int32)
     ip = np.zeros(16, dtype=np.int32)
-    args = (C.byref(a.mat), p.ctypes.data_as(C.POINTER(C.c_int)), ip.ctypes.data_as(C.POINTER(C.c_int)))
-    assert P.fn("CsrAmdOrder")(*args) == -1
-    assert P.fn("CsrNdOrder")(*args) == -1
+    pp = p.ctypes.data_as(C.POINTER(C.c_int))
+    args = (C.byref(a.mat), pp, ip.ctypes.data_as(C.POINTER(C.c_int)))
+    assert P.fn("CsrAmdOrder")(*args) == 0 and np.array_equal(ip[p], np.arange(16))
+    assert P.fn("CsrNdOrder")(*args) == 1 and np.array_equal(ip[p], np.arange(16))
+    assert P.fn("CsrAmdOrder")(C.byref(a.mat), pp, None) == 0
+    assert P.fn("CsrAmdOrder")(C.byref(a.mat), None, None) == -1
+    assert P.fn("CsrNdOrder")(C.byref(a.mat), None, None) == 0
 
 
 def test_handles_and_defaults_without_device():
