@@ -94,12 +94,15 @@ public:
     // barrier per local level, 3 the same with the block's x held in LDS as
     // value-flags instead of barriers, 4 one workgroup walking the level
     // schedule with a barrier per level (no hand-offs between workgroups;
-    // for deep schedules with few rows per level).
+    // for deep schedules with few rows per level), 5 block-local with the
+    // work of a block split over a solver wave (LDS only), a wave streaming
+    // the block's records into an LDS ring and a wave importing its
+    // out-of-block values (1-16 lanes per row).
     void SetSweepMode(int mode_l, int mode_u, int blk_rows);
     // Declares that row x + nx (y + ny z) is grid cell (x, y, z); enables
     // tiled partitions of the block-local sweeps.
     void SetGridShape(int nx, int ny, int nz);
-    // Block-local sweeps (mode 0, 2 or 3 per direction) over tx x ty x tz
+    // Block-local sweeps (mode 0, 2, 3 or 5 per direction) over tx x ty x tz
     // tiles of the declared grid (at most 16384 rows per tile).  Blocks run
     // in a topological order of their dependency graph; the results are
     // those of the same lanes per row with any other partition.

@@ -58,9 +58,9 @@ int IluSolverAutotuneEx(IluSolverHdl hdl, int reps, double* table, int cap);
  * software-pipelined, with a workgroup barrier per local level, 3 the same
  * with the block's x as LDS value-flags and no barriers, 4 one workgroup
  * walking the level schedule with a barrier per level, for deep schedules
- * with few rows per level; blk_rows is ignored by modes 0 and 4) and the
- * current
- * configuration (cfg[8]: mode_l,
+ * with few rows per level, 5 block-local with one solver, one record-stream
+ * loader and one importer wave per block, 1-16 lanes per row; blk_rows is
+ * ignored by modes 0 and 4) and the current configuration (cfg[8]: mode_l,
  * lanes_l, mode_u, lanes_u, blk_rows, tile x, tile y, tile z). */
 #define IluSolverSetSweepModeEx LLJBASH_DECL(IluSolverSetSweepModeEx)
 int IluSolverSetSweepModeEx(IluSolverHdl hdl, int mode_l, int mode_u, int blk_rows);
