@@ -2,7 +2,7 @@
 //
 // Same partition and schedule as the block-local sweeps (blocks processed
 // in a topological order of the block graph, rows of a block in local
-// level order), but each workgroup splits the work of one block over three
+// level order), but each workgroup splits the work of one block over four
 // waves so that no wave waits on memory requests issued for another
 // purpose:
 //   * wave 0 solves.  Its inner loop touches LDS only: the block's x (b on
@@ -17,10 +17,13 @@
 //     into a two-group LDS ring in groups of eight 1 KiB chunks, four
 //     groups in flight, flow-controlled by two LDS counters (bytes loaded /
 //     bytes consumed).
-//   * wave 2 imports: it polls the face buffer for the block's out-of-block
-//     dependencies, in order of first use and at most kWsLook records ahead
-//     of the solver, and writes each value into its LDS slot as a
-//     value-flag (kPending until imported).
+//   * waves 2 and 3 import: each polls the face buffer for the block's
+//     out-of-block dependencies, in order of first use and at most kWsLook
+//     records ahead of the solver, and writes each value into its LDS slot
+//     as a value-flag (kPending until imported).  The two poll the same
+//     imports half a poll round apart, which shortens the time from a
+//     producer's publish to the import (a poll round is one memory round
+//     trip).
 // Row sums follow the lane order of every other sweep kind (lane t of a
 // row's group accumulates entries t, t+G, ... with fma, lane 0 from b_i,
 // then the group butterfly), so results are bit-identical to k_sweep with
@@ -45,7 +48,7 @@
 
 namespace pilu {
 
-constexpr int kWsThreads = 192;   // solver, loader and importer waves
+constexpr int kWsThreads = 256;   // solver, loader and two importer waves
 constexpr int kWsChunk = 1024;    // bytes the loader moves per step (64 lanes x 16 B)
 constexpr int kWsRing = 16384;    // LDS ring of the record stream (bytes)
 constexpr int kWsGroup = 8;       // chunks the loader writes per ring-space check (the ring holds two groups)
@@ -340,7 +343,8 @@ __device__ __forceinline__ void ws_load(const WsBlock& bk, const uint4* __restri
     }
 }
 
-// Wave 2: import the block's out-of-block dependencies into their slots.
+// Waves 2 and 3: import the block's out-of-block dependencies into their
+// slots (both write the same value when both see it).
 // imports: the block's (face slot, first record) list, staged in LDS, so a
 // lane moves on to its next import without a memory round trip.
 __device__ __forceinline__ void ws_import(const WsBlock& bk, const int2* imports, double* imp,
@@ -456,7 +460,11 @@ __global__ __launch_bounds__(kWsThreads) void k_sweep_ws(const WsBlock* __restri
         __builtin_amdgcn_s_setprio(0);
     }
     else if (wave == 1) ws_load(bk, stream, reinterpret_cast<uint4*>(ring), ctl);
-    else ws_import(bk, imp_ent, imp, face, ctl, wd, timed_out);
+    else {
+        // the second importer starts about half a poll round later
+        if (wave == 3) __builtin_amdgcn_s_sleep(10);
+        ws_import(bk, imp_ent, imp, face, ctl, wd, timed_out);
+    }
     if (timed_out) atomicOr(err, kErrTimeout);
     if (bad) atomicOr(err, kErrStream);
     retire_ticket(ctr);
