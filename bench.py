@@ -480,15 +480,17 @@ def traffic_bytes(path, form):
 def cpu_baseline(a, b, tol, max_threads):
     """The reference's algorithm on the host cores, timed over complete
     solves: the oracle's OpenMP build (oracle/baseline_omp.c: the same
-    per-row arithmetic, the rows of one dependency level in parallel, a
-    stand-in for the reference's subtree/queue schedule; threaded SpMV and
-    BLAS-1; the reference's release flags) runs the phases of
-    test/benchmark.cpp:104-131 -- schedule (setup), factorization, and
+    per-row arithmetic; the reference's subtree / queue schedule of
+    src/subtree.hpp restated, set up as src/ilu_solver.cpp:295-345 does;
+    threaded SpMV and BLAS-1; the reference's release flags) runs the phases
+    of test/benchmark.cpp:104-131 -- schedule (setup), factorization, and
     ILU(1)-GMRES(50) from x0 = 0 to the same tolerance as the GPU -- with
     every thread the process may use (at most max_threads), with 8 (the
     reference caps its sweeps at 8 threads) and with one thread (the
-    reference benchmark's default -t 1).  The symbolic pass (untimed in the
-    reference's benchmark) is timed once, single-threaded."""
+    reference benchmark's default -t 1).  The same build on level sets
+    (sched 0), which is faster on this grid, is timed once at the most
+    threads for context.  The symbolic pass (untimed in the reference's
+    benchmark) is timed once, single-threaded."""
     import numpy as np
 
     from oracle import py_oracle as O
@@ -497,20 +499,25 @@ def cpu_baseline(a, b, tol, max_threads):
     frp, fci, flev, fval, nz = O.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1, fast=True)
     t_sym = time.perf_counter() - t0
     threads = min(O.baseline_threads(), max(1, max_threads))
-    runs = {}
-    for th in sorted({threads, min(threads, 8), 1}, reverse=True):
+    bnorm = float(np.linalg.norm(b))
+
+    def solve(th, sched):
         O.baseline_threads(th)
         t0 = time.perf_counter()
         ret, its, x, times = O.baseline_ilu_gmres((a.row_ptr, a.col_idx, a.value), (frp, fci, fval), b, tol, 50,
-                                                  2000)
+                                                  2000, sched=sched)
         wall = time.perf_counter() - t0
-        res = float(np.linalg.norm(b - a.matvec(x)))
-        runs[th] = {"setup_s": times[0], "factorize_s": times[1], "gmres_s": times[2],
-                    "solve_s": times[1] + times[2], "end_to_end_s": t_sym + times[0] + times[1] + times[2],
-                    "its": its, "converged": bool(ret == 1), "true_rel_res": res / float(np.linalg.norm(b)),
-                    "wall_s": wall}
-        log(f"[bench] cpu {th} threads: setup {times[0]:.3f}s factorize {times[1]:.3f}s gmres {times[2]:.2f}s "
-            f"({its} steps, true rel res {runs[th]['true_rel_res']:.2e})")
+        r = {"setup_s": times[0], "factorize_s": times[1], "gmres_s": times[2],
+             "solve_s": times[1] + times[2], "end_to_end_s": t_sym + times[0] + times[1] + times[2],
+             "its": its, "converged": bool(ret == 1),
+             "true_rel_res": float(np.linalg.norm(b - a.matvec(x))) / bnorm, "wall_s": wall}
+        log(f"[bench] cpu {th} threads, {'subtree/queue' if sched else 'level sets'}: setup {times[0]:.3f}s "
+            f"factorize {times[1]:.3f}s gmres {times[2]:.2f}s ({its} steps, true rel res {r['true_rel_res']:.2e})")
+        return r
+
+    runs = {th: solve(th, 1) for th in sorted({threads, min(threads, 8), 1}, reverse=True)}
+    levels = solve(threads, 0)
+    info = O.baseline_subtree_info(frp, fci, check=False)  # on `threads` threads
     O.baseline_threads(threads)
     best = runs[threads]
     return {
@@ -518,13 +525,17 @@ def cpu_baseline(a, b, tol, max_threads):
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one complete ILU(1)-GMRES(50) solve of the same 128^3 system to the same tolerance "
+        "sample": f"one complete ILU(1)-GMRES(50) solve of the same system (n = {a.n}) to the same tolerance "
                   f"(factorization + {best['its']} GMRES steps, all timed, none extrapolated) on {threads} host "
-                  f"threads: the oracle's OpenMP build, level-scheduled rows standing in for the reference's "
-                  f"subtree/queue schedule, threaded SpMV/BLAS-1, -O3 -ffast-math; value = factorize + GMRES "
+                  f"threads: the oracle's OpenMP build with the reference's subtree/queue schedule restated "
+                  f"(elimination-forest subtrees per thread, stripped first levels, the rest queued: "
+                  f"{info['queue']['l']} of {a.n} rows per sweep run on one thread, as the reference does for rows "
+                  f"of fewer than 40 entries), threaded SpMV/BLAS-1, -O3 -ffast-math; value = factorize + GMRES "
                   f"(the GPU step's phases), end_to_end_s adds symbolic and schedule setup",
         "end_to_end_s": best["end_to_end_s"],
         "by_threads": {str(k): v for k, v in runs.items()},
+        "level_sets": {"threads": threads, **levels},
+        "schedule": info,
         "symbolic_s": t_sym,
     }
 

@@ -178,7 +178,9 @@ def _load_omp() -> C.CDLL:
         lib.baseline_set_threads.argtypes = [C.c_int]
         lib.baseline_ilu_gmres.restype = C.c_int
         lib.baseline_ilu_gmres.argtypes = [C.c_int, _I, _I, _D, _I, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int,
-                                           C.POINTER(C.c_int), _D]
+                                           C.POINTER(C.c_int), _D, C.c_int]
+        lib.baseline_subtree_info.restype = C.c_int
+        lib.baseline_subtree_info.argtypes = [C.c_int, _I, _I, _I, _I, C.c_int]
         _OMP = lib
     return _OMP
 
@@ -191,8 +193,9 @@ def baseline_threads(k: int | None = None) -> int:
     return int(lib.baseline_threads())
 
 
-def baseline_ilu_gmres(a, factors_pattern, rhs, tol, m, maxits):
-    """OpenMP level-scheduled ILU + GMRES(m) (baseline_omp.c).
+def baseline_ilu_gmres(a, factors_pattern, rhs, tol, m, maxits, sched: int = 1, return_factors: bool = False):
+    """OpenMP ILU + GMRES(m) (baseline_omp.c); sched 1: the reference's
+    subtree / queue schedule, 0: level sets.
     factors_pattern = (frp, fci, fval before factorization).
     -> (ret, its, x, times[schedule, factorize, gmres])."""
     lib = _load_omp()
@@ -204,5 +207,25 @@ def baseline_ilu_gmres(a, factors_pattern, rhs, tol, m, maxits):
     its = C.c_int(0)
     times = np.zeros(3)
     ret = lib.baseline_ilu_gmres(arp.size - 1, _pi(arp), _pi(aci), _pd(aval), _pi(frp), _pi(fci), _pi(_i(dg)),
-                                 _pd(fval), _pd(b), _pd(x), float(tol), int(m), int(maxits), C.byref(its), _pd(times))
+                                 _pd(fval), _pd(b), _pd(x), float(tol), int(m), int(maxits), C.byref(its), _pd(times),
+                                 int(sched))
+    if return_factors:
+        return ret, its.value, x, times, fval
     return ret, its.value, x, times
+
+
+def baseline_subtree_info(frp, fci, check: bool = True):
+    """Shape of the subtree / queue schedule (baseline_omp.c) on the current
+    threads: dict of queued rows and stripped levels per schedule, whether
+    the sweeps queue in parallel, and (check) whether every row appears once
+    and no subtree row depends on another thread's or a queued row."""
+    lib = _load_omp()
+    rp, ci = _i(frp), _i(fci)
+    dg = _i(diag_ptr(rp, ci))
+    info = np.zeros(7, dtype=np.int32)
+    rc = lib.baseline_subtree_info(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pi(info), int(check))
+    if rc < 0:
+        raise RuntimeError("baseline_subtree_info failed")
+    return dict(ok=bool(rc) if check else None, queue=dict(fact=int(info[0]), l=int(info[1]), u=int(info[2])),
+                stripped_levels=dict(fact=int(info[3]), l=int(info[4]), u=int(info[5])),
+                parallel_sweep_queue=bool(info[6]))

@@ -101,10 +101,13 @@ def test_solution_matches_direct_solve(oracle):
     assert np.linalg.norm(x - ref) <= 1e-7 * np.linalg.norm(ref)
 
 
-def test_openmp_baseline_matches_serial_oracle(oracle):
-    """The threaded CPU baseline (level-scheduled ILU + GMRES) solves the
-    same systems as the serial oracle: same factors (identical per-row
-    arithmetic), iterations within one, true residual at the tolerance."""
+@pytest.mark.parametrize("sched", [0, 1])
+def test_openmp_baseline_matches_serial_oracle(oracle, sched):
+    """The threaded CPU baseline (ILU + GMRES on level sets, sched 0, or on
+    the reference's subtree / queue schedule, sched 1) solves the same
+    systems as the serial oracle: the same factors bit for bit (identical
+    per-row arithmetic in a dependency-respecting order), iterations within
+    one, true residual at the tolerance."""
     from oracle import py_oracle as O
 
     a = gen.convdiff3d(12)
@@ -115,7 +118,49 @@ def test_openmp_baseline_matches_serial_oracle(oracle):
     ok_s, its_s, _, _ = oracle.gmres((a.row_ptr, a.col_idx, a.value), b, tol, 30, 500, factors=(frp, fci, dg, fac))
     for threads in (1, 4):
         O.baseline_threads(threads)
-        ret, its, x, times = O.baseline_ilu_gmres((a.row_ptr, a.col_idx, a.value), (frp, fci, fval), b, tol, 30, 500)
+        ret, its, x, times, f = O.baseline_ilu_gmres((a.row_ptr, a.col_idx, a.value), (frp, fci, fval), b, tol, 30,
+                                                     500, sched=sched, return_factors=True)
         assert ret == 1 and ok_s and abs(its - its_s) <= 1
+        assert np.array_equal(f, fac)
         assert np.linalg.norm(b - a.matvec(x)) <= 1.01 * tol
         assert np.all(times >= 0)
+
+
+def _forest(k, side):
+    from par_ilu0_cpp_amd import block_diagonal
+
+    rp, ci, va, _ = block_diagonal([gen.poisson2d(side) for _ in range(k)])
+    return rp, ci, va
+
+
+@pytest.mark.parametrize("threads", [2, 3, 8])
+def test_subtree_schedule_is_valid(oracle, threads):
+    """The subtree / queue schedule of the baseline (the reference's
+    src/subtree.hpp): every row appears exactly once in each of the three
+    schedules, and no row of a thread's subtrees depends on another thread's
+    rows or on a queued row, on grids (a chain-like elimination tree: mostly
+    stripped levels and queue), on rows with >= 40 entries (parallel sweep
+    queue), and on a forest of independent systems, where whole systems
+    become subtrees and almost nothing is queued."""
+    from oracle import py_oracle as O
+    from tests.conftest import random_pattern
+
+    O.baseline_threads(threads)
+    cases = []
+    a = gen.poisson2d(40)
+    cases.append(oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 0)[:2])
+    a = gen.convdiff3d(12)
+    cases.append(oracle.iluk_symbolic(a.row_ptr, a.col_idx, a.value, 1)[:2])
+    rp, ci = random_pattern(2000, 50, 400, seed=5)
+    cases.append((rp, ci))
+    rp, ci, va = _forest(24, 12)
+    forest = oracle.iluk_symbolic(rp, ci, va, 1)[:2]
+    cases.append(forest)
+    for frp, fci in cases:
+        info = O.baseline_subtree_info(frp, fci)
+        assert info["ok"], info
+    dense = O.baseline_subtree_info(*cases[2])
+    assert dense["parallel_sweep_queue"] == (threads > 1)
+    info = O.baseline_subtree_info(*forest)
+    n = len(forest[0]) - 1
+    assert max(info["queue"].values()) <= 0.1 * n, info
