@@ -195,6 +195,18 @@ struct Watchdog {
 // Ticks of the device wall clock in `seconds` (host side).
 uint64_t watchdog_ticks(double seconds);
 
+// Host-side phase times of the set-up path, printed to stderr when the
+// environment holds PAR_ILU_TIMING (each mark first waits for `stream`).
+bool phase_timing_enabled();
+struct PhaseTimer {
+    const char* scope;
+    hipStream_t stream;
+    bool on;
+    double t0;
+    PhaseTimer(const char* sc, hipStream_t s);
+    void mark(const char* what);
+};
+
 inline int ceil_div(long a, long b) { return static_cast<int>((a + b - 1) / b); }
 
 }  // namespace pilu

@@ -3,9 +3,12 @@
 // Interface: include/ilu_solver.hpp (reference include/ilu_solver.hpp:8-30,
 // behaviour src/ilu_solver.cpp:225-831).
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -70,6 +73,26 @@ void check_sweep_errors(lljbash::IluSolver::Ext* e) {
 }
 
 double* device_scratch();
+
+bool phase_timing_enabled() {
+    static const bool on = std::getenv("PAR_ILU_TIMING") != nullptr;
+    return on;
+}
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+PhaseTimer::PhaseTimer(const char* sc, hipStream_t s)
+    : scope(sc), stream(s), on(phase_timing_enabled()), t0(on ? now_ms() : 0.0) {}
+
+void PhaseTimer::mark(const char* what) {
+    if (!on) return;
+    (void)hipStreamSynchronize(stream);
+    const double t = now_ms();
+    std::fprintf(stderr, "[par-ilu timing] %s: %s %.2f ms\n", scope, what, t - t0);
+    t0 = t;
+}
 
 // Raised when a sweep structure is missing during a no-allocation apply
 // (building it would allocate device memory).
@@ -266,39 +289,88 @@ void build_block_schedule(Ext* e, bool upper, const int* block_of, int nblk, con
 // chain), ties by block id: a topological order that starts the blocks of
 // one wavefront together.  Throws if the block graph has a cycle (the
 // partition cannot be swept block by block).  Returns pos[block].
-std::vector<int> block_order(const Ext* e, const std::vector<int>& rp, const std::vector<int>& ci,
-                             const std::vector<int>& dg, bool upper) {
+// Cross-block dependencies of one direction: per row, its entries whose
+// column lies in another block (count pass, then keys (block of the row,
+// block of the column) in 64 bits).
+__global__ void k_cross_count(int n, const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ dg,
+                              const int* __restrict__ bo, int upper, int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = bo[i], p0 = upper ? dg[i] + 1 : rp[i], p1 = upper ? rp[i + 1] : dg[i];
+    int c = 0;
+    for (int p = p0; p < p1; ++p) c += bo[ci[p]] != b;
+    cnt[i] = c;
+}
+
+__global__ void k_cross_write(int n, const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ dg,
+                              const int* __restrict__ bo, int upper, const int* __restrict__ off,
+                              unsigned long long* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = bo[i], p0 = upper ? dg[i] + 1 : rp[i], p1 = upper ? rp[i + 1] : dg[i];
+    int k = off[i];
+    for (int p = p0; p < p1; ++p) {
+        const int c = bo[ci[p]];
+        if (c != b) keys[k++] = (static_cast<unsigned long long>(static_cast<unsigned>(b)) << 32) | static_cast<unsigned>(c);
+    }
+}
+
+// Processing order of the blocks of a partition in one direction: a row
+// depends on the rows of its L (forward) or U (backward) part, so block bj
+// must run before block bi when a row of bi depends on a row of bj.  The
+// blocks are ordered by their depth in that graph (longest dependency
+// chain), ties by block id: a topological order that starts the blocks of
+// one wavefront together.  The distinct block edges are found on the
+// device (cross-block entries, sorted and deduplicated); Kahn's algorithm
+// runs on the host over them.  Throws if the block graph has a cycle (the
+// partition cannot be swept block by block).  Returns pos[block].
+std::vector<int> block_order(Ext* e, bool upper) {
     const int n = e->n, nblk = e->nblk;
-    const std::vector<int>& bo = e->h_block_of;
-    // rows grouped by block (counting sort)
-    std::vector<int> start(static_cast<size_t>(nblk) + 1, 0), rows(n);
-    for (int i = 0; i < n; ++i) ++start[bo[i] + 1];
-    for (int b = 0; b < nblk; ++b) start[b + 1] += start[b];
-    std::vector<int> fill(start.begin(), start.end() - 1);
-    for (int i = 0; i < n; ++i) rows[fill[bo[i]]++] = i;
-    // predecessor lists, deduplicated with a marker per block (blocks in
-    // parallel, each thread with its own marker array)
-    std::vector<std::vector<int>> pl(static_cast<size_t>(nblk));
-    parallel_for(nblk, [&](long lo, long hi) {
-        std::vector<int> mark(nblk, -1);
-        for (long b = lo; b < hi; ++b) {
-            for (int k = start[b]; k < start[b + 1]; ++k) {
-                const int i = rows[k];
-                const int e0 = upper ? dg[i] + 1 : rp[i], e1 = upper ? rp[i + 1] : dg[i];
-                for (int p = e0; p < e1; ++p) {
-                    const int bj = bo[ci[p]];
-                    if (bj != b && mark[bj] != b) {
-                        mark[bj] = static_cast<int>(b);
-                        pl[b].push_back(bj);
-                    }
-                }
-            }
-        }
-    });
+    hipStream_t s = e->stream;
+    DevBuf<int> cnt(static_cast<size_t>(n) + 1), off(static_cast<size_t>(n) + 1);
+    k_fill_int<<<1, 1, 0, s>>>(1, cnt.get() + n, 0);
+    k_cross_count<<<grid_for(n, 256), 256, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), e->block_of.get(),
+                                                  upper ? 1 : 0, cnt.get());
+    PILU_HIP(hipGetLastError());
+    size_t bytes = 0;
+    PILU_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt.get(), off.get(), 0, static_cast<size_t>(n) + 1,
+                                     rocprim::plus<int>(), s));
+    e->sort_tmp.reserve(bytes);
+    PILU_HIP(rocprim::exclusive_scan(e->sort_tmp.get(), bytes, cnt.get(), off.get(), 0, static_cast<size_t>(n) + 1,
+                                     rocprim::plus<int>(), s));
+    const int total = read_int(off.get() + n, s);
     std::vector<int> pred_ptr(static_cast<size_t>(nblk) + 1, 0), pred;
-    for (int b = 0; b < nblk; ++b) {
-        pred.insert(pred.end(), pl[b].begin(), pl[b].end());
-        pred_ptr[b + 1] = static_cast<int>(pred.size());
+    if (total > 0) {
+        DevBuf<unsigned long long> keys(total), sorted(total), uniq(total);
+        DevBuf<unsigned> nuniq(1);
+        k_cross_write<<<grid_for(n, 256), 256, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), e->block_of.get(),
+                                                      upper ? 1 : 0, off.get(), keys.get());
+        PILU_HIP(hipGetLastError());
+        int pbits = 1;
+        while ((1 << pbits) < nblk) ++pbits;
+        bytes = 0;
+        PILU_HIP(rocprim::radix_sort_keys(nullptr, bytes, keys.get(), sorted.get(), static_cast<unsigned>(total), 0u,
+                                          static_cast<unsigned>(32 + pbits), s));
+        e->sort_tmp.reserve(bytes);
+        PILU_HIP(rocprim::radix_sort_keys(e->sort_tmp.get(), bytes, keys.get(), sorted.get(),
+                                          static_cast<unsigned>(total), 0u, static_cast<unsigned>(32 + pbits), s));
+        bytes = 0;
+        PILU_HIP(rocprim::unique(nullptr, bytes, sorted.get(), uniq.get(), nuniq.get(), static_cast<size_t>(total),
+                                 rocprim::equal_to<unsigned long long>(), s));
+        e->sort_tmp.reserve(bytes);
+        PILU_HIP(rocprim::unique(e->sort_tmp.get(), bytes, sorted.get(), uniq.get(), nuniq.get(),
+                                 static_cast<size_t>(total), rocprim::equal_to<unsigned long long>(), s));
+        unsigned nu = 0;
+        PILU_HIP(hipMemcpyAsync(&nu, nuniq.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        PILU_HIP(hipStreamSynchronize(s));
+        std::vector<unsigned long long> edges(nu);
+        PILU_HIP(hipMemcpy(edges.data(), uniq.get(), sizeof(unsigned long long) * nu, hipMemcpyDeviceToHost));
+        pred.resize(nu);
+        for (unsigned k = 0; k < nu; ++k) {
+            ++pred_ptr[static_cast<int>(edges[k] >> 32) + 1];
+            pred[k] = static_cast<int>(edges[k] & 0xffffffffu);
+        }
+        for (int b = 0; b < nblk; ++b) pred_ptr[b + 1] += pred_ptr[b];
     }
     // successor lists and Kahn's algorithm, computing depths on the way
     std::vector<int> indeg(nblk), succ_ptr(static_cast<size_t>(nblk) + 1, 0), succ(pred.size());
@@ -338,6 +410,7 @@ std::vector<int> block_order(const Ext* e, const std::vector<int>& rp, const std
 // order, which k_sweep_blk relies on.
 void install_partition(Ext* e, int nblk, bool contiguous) {
     const int n = e->n;
+    PhaseTimer pt("partition", e->stream);
     e->nblk = nblk;
     std::vector<int> cnt(nblk, 0);
     for (int i = 0; i < n; ++i) ++cnt[e->h_block_of[i]];
@@ -353,12 +426,16 @@ void install_partition(Ext* e, int nblk, bool contiguous) {
             pos_u[b] = nblk - 1 - b;
         }
     } else {
-        host_pattern(e);
-        pos_l = block_order(e, e->h_rp, e->h_ci, e->h_dg, false);
-        pos_u = block_order(e, e->h_rp, e->h_ci, e->h_dg, true);
+        pt.mark("upload");
+        pos_l = block_order(e, false);
+        pt.mark("L block order");
+        pos_u = block_order(e, true);
+        pt.mark("U block order");
     }
     build_block_schedule(e, false, e->block_of.get(), e->nblk, pos_l, e->bperm_l, e->blev_l, e->loff_l);
+    pt.mark("L block schedule");
     build_block_schedule(e, true, e->block_of.get(), e->nblk, pos_u, e->bperm_u, e->blev_u, e->loff_u);
+    pt.mark("U block schedule");
     ++e->part_version;  // packs are rebuilt on first use
 }
 
@@ -371,6 +448,48 @@ void use_blocks(Ext* e, int blk) {
     e->blk_rows = blk;
     e->tile[0] = e->tile[1] = e->tile[2] = 0;
     install_partition(e, ceil_div(e->n, blk), true);
+}
+
+// Skew factors of the tiled partition from the L entries' grid offsets
+// (dx, dy, dz) (a row at x + nx (y + ny z)): pass 0 gives b from the
+// same-plane entries with dy < 0 (ceil(dx / -dy)) and c2 from the entries
+// with dz < 0 (ceil(dy / -dz)); pass 1 gives c (ceil((dx + b dy) / -dz)).
+// Maxima over all entries, at least 0, in out[0] (b, c) and out[1] (c2).
+__device__ __forceinline__ long skew_ceil_div(long a, long b) { return a >= 0 ? (a + b - 1) / b : -((-a) / b); }
+
+__global__ void k_skew(int n, long nx, long ny, const int* __restrict__ rp, const int* __restrict__ ci,
+                       const int* __restrict__ dg, int pass, long b, unsigned long long* __restrict__ out) {
+    __shared__ long red[2][256];
+    long l0 = 0, l1 = 0;
+    const long nxy = nx * ny;
+    for (long r = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; r < n;
+         r += static_cast<long>(gridDim.x) * blockDim.x) {
+        const long x = r % nx, y = (r / nx) % ny, z = r / nxy;
+        for (int p = rp[r]; p < dg[r]; ++p) {
+            const long q = ci[p];
+            const long dx = q % nx - x, dy = (q / nx) % ny - y, dz = q / nxy - z;
+            if (pass == 0) {
+                if (dz == 0 && dy < 0) l0 = max(l0, skew_ceil_div(dx, -dy));
+                if (dz < 0) l1 = max(l1, skew_ceil_div(dy, -dz));
+            } else if (dz < 0) {
+                l0 = max(l0, skew_ceil_div(dx + b * dy, -dz));
+            }
+        }
+    }
+    red[0][threadIdx.x] = l0;
+    red[1][threadIdx.x] = l1;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) {
+            red[0][threadIdx.x] = max(red[0][threadIdx.x], red[0][threadIdx.x + h]);
+            red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMax(&out[0], static_cast<unsigned long long>(red[0][0]));
+        atomicMax(&out[1], static_cast<unsigned long long>(red[1][0]));
+    }
 }
 
 // Tiles of the grid declared by SetGridShape (row x + nx (y + ny z)).
@@ -390,48 +509,29 @@ void use_tiles(Ext* e, int tx, int ty, int tz) {
         throw std::runtime_error("tile must hold 1..16384 rows");
     if (e->tile[0] == tx && e->tile[1] == ty && e->tile[2] == tz) return;
     const int n = e->n;
-    host_pattern(e);
-    const std::vector<int>& rp = e->h_rp;
-    const std::vector<int>& dg = e->h_dg;
-    const std::vector<int>& ci = e->h_ci;
-    const GridCoord gc(nx, ny);
-    auto ceil_div_signed = [](long a, long b) { return a >= 0 ? (a + b - 1) / b : -((-a) / b); };
+    PhaseTimer pt("tiles", e->stream);
+    hipStream_t st = e->stream;
     long b = 0, c = 0, c2 = 0;
-    for (int pass = 0; pass < 2; ++pass) {  // b first, then c (which depends on b)
-        std::mutex mu;
-        parallel_for(n, [&](long lo, long hi) {
-            long lb = 0, lc = 0, lc2 = 0;
-            long x, y, z;
-            gc(lo, x, y, z);
-            for (long r = lo; r < hi; ++r) {
-                for (int p = rp[r]; p < dg[r]; ++p) {
-                    long qx, qy, qz;
-                    gc(ci[p], qx, qy, qz);
-                    const long dx = qx - x, dy = qy - y, dz = qz - z;
-                    if (pass == 0) {
-                        if (dz == 0 && dy < 0) lb = std::max(lb, ceil_div_signed(dx, -dy));
-                        if (dz < 0) lc2 = std::max(lc2, ceil_div_signed(dy, -dz));
-                    } else if (dz < 0) {
-                        lc = std::max(lc, ceil_div_signed(dx + b * dy, -dz));
-                    }
-                }
-                if (++x == nx) {
-                    x = 0;
-                    if (++y == ny) {
-                        y = 0;
-                        ++z;
-                    }
-                }
-            }
-            std::lock_guard<std::mutex> g(mu);
+    {
+        DevBuf<unsigned long long> mx(2);
+        const int grid = std::min(grid_for(n, 256), 4096);
+        for (int pass = 0; pass < 2; ++pass) {  // b first, then c (which depends on b)
+            PILU_HIP(hipMemsetAsync(mx.get(), 0, 2 * sizeof(unsigned long long), st));
+            k_skew<<<grid, 256, 0, st>>>(n, nx, ny, e->rp.get(), e->ci.get(), e->dg.get(), pass, b, mx.get());
+            PILU_HIP(hipGetLastError());
+            unsigned long long r[2] = {0, 0};
+            PILU_HIP(hipMemcpyAsync(r, mx.get(), sizeof(r), hipMemcpyDeviceToHost, st));
+            PILU_HIP(hipStreamSynchronize(st));
             if (pass == 0) {
-                b = std::max(b, lb);
-                c2 = std::max(c2, lc2);
+                b = static_cast<long>(r[0]);
+                c2 = static_cast<long>(r[1]);
             } else {
-                c = std::max(c, lc);
+                c = static_cast<long>(r[0]);
             }
-        });
+        }
     }
+    pt.mark("skew");
+    const GridCoord gc(nx, ny);
     const long umax = (nx - 1) + b * (ny - 1) + c * (nz - 1), vmax = (ny - 1) + c2 * (nz - 1);
     const long ntu = umax / tx + 1, ntv = vmax / ty + 1, ntw = (nz - 1) / tz + 1;
     if (ntu * ntv * ntw > (1L << 30)) throw std::runtime_error("tiled sweeps: too many tiles");
@@ -467,12 +567,14 @@ void use_tiles(Ext* e, int tx, int ty, int tz) {
     e->skew[0] = static_cast<int>(b);
     e->skew[1] = static_cast<int>(c);
     e->skew[2] = static_cast<int>(c2);
+    pt.mark("tile of every row");
     install_partition(e, nblk, false);
 }
 
 // Device pattern is in e->rp / e->ci: validate, locate diagonals, schedule.
 void analyse(Ext* e, const int* host_rp) {
     const int n = e->n;
+    PhaseTimer pt("analyse", e->stream);
     e->analysed = false;  // until every schedule below is built
     e->factored = false;
     hipStream_t s = e->stream;
@@ -515,10 +617,13 @@ void analyse(Ext* e, const int* host_rp) {
     e->max_row = max_row;
     e->max_lo = max_lo;
     e->max_up = max_up;
+    pt.mark("pattern checks, diagonal, host row pointers");
 
     (void)device_scratch();  // the no-allocation apply's scratch on this device
     e->nlev_l = build_schedule(e, false, e->perm_l, e->lvoff_l);
+    pt.mark("L level schedule");
     e->nlev_u = build_schedule(e, true, e->perm_u, e->lvoff_u);
+    pt.mark("U level schedule");
     ++e->pattern_version;
 
     e->vin.reserve(e->nnz);
@@ -745,8 +850,15 @@ void gather_pack(Ext* e, SweepPack& pk, bool upper) {
 
 void ensure_packs(Ext* e) {
     if (e->nblk <= 0) throw std::runtime_error("packed sweep needs a block partition");
-    if (e->pack_l.version != e->part_version) build_pack(e, false, e->pack_l);
-    if (e->pack_u.version != e->part_version) build_pack(e, true, e->pack_u);
+    PhaseTimer pt("packs", e->stream);
+    if (e->pack_l.version != e->part_version) {
+        build_pack(e, false, e->pack_l);
+        pt.mark("L pack");
+    }
+    if (e->pack_u.version != e->part_version) {
+        build_pack(e, true, e->pack_u);
+        pt.mark("U pack");
+    }
     if (e->factored) {
         if (e->pack_l.factors != e->factor_version) gather_pack(e, e->pack_l, false);
         if (e->pack_u.factors != e->factor_version) gather_pack(e, e->pack_u, true);
@@ -1648,8 +1760,10 @@ int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx,
     Ext* e = ext_;
     e->analysed = false;  // the previous pattern is gone whether or not this completes
     pilu::SymbolicResult res;
+    pilu::PhaseTimer pt("symbolic", e->stream);
     const int nnz_f = pilu::symbolic_device(n, d_row_ptr, d_col_idx, nullptr, lof, res, e->stream,
                                             2.0 * e->watchdog_s);
+    pt.mark("symbolic ILU(k)");
     if (nnz_f < 0) return -1;
     int nnz_a = 0;
     PILU_HIP(hipMemcpy(&nnz_a, d_row_ptr + n, sizeof(int), hipMemcpyDeviceToHost));

@@ -283,6 +283,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     if (n >= (1 << 27) - 1) throw std::runtime_error("IlukSymbolic: n must be below 2^27 - 1");
     if (lof < 0 || lof > 31) throw std::runtime_error("IlukSymbolic: level of fill must be in [0, 31]");
     int nnz_a = 0;
+    PhaseTimer pt("symbolic pass", s);
     check(hipMemcpyAsync(&nnz_a, d_arp + n, sizeof(int), hipMemcpyDeviceToHost, s), "nnz");
     check(hipStreamSynchronize(s), "sync");
     DevBuf<uint64_t> header(n);
@@ -330,6 +331,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
         }
         break;
     }
+    pt.mark("pool fill and k_symbolic");
     // row_ptr = exclusive scan of the row counts.
     out.n = n;
     out.rp.resize(static_cast<size_t>(n) + 1);
@@ -350,6 +352,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     out.lev.resize(std::max(nnz_f, 1));
     out.val.resize(std::max(nnz_f, 1));
     out.nzmap.resize(std::max(nnz_a, 1));
+    pt.mark("scan and allocation of F");
     check(hipMemsetAsync(misc.get() + 2, 0, sizeof(int), s), "err reset");
     k_sym_emit<<<ceil_div(n, 4), 256, 0, s>>>(n, d_arp, d_aval, pool.get(), row_off.get(), out.rp.get(), out.ci.get(),
                                               out.lev.get(), d_aval ? out.val.get() : nullptr, out.nzmap.get(),
@@ -359,6 +362,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     check(hipMemcpyAsync(&err, misc.get() + 2, sizeof(int), hipMemcpyDeviceToHost, s), "err");
     check(hipStreamSynchronize(s), "sync");
     if (err) return -1;
+    pt.mark("k_sym_emit");
     return nnz_f;
 }
 
