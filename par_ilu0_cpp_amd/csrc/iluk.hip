@@ -12,6 +12,15 @@
 // per-row header; both are value-as-flag words, so a waiting row spins on
 // exactly the words it reads.  A compaction pass then lays the rows out as
 // CSR with the reference's value placement and nzmap.
+//
+// Row order.  A row waits only for its pivot rows, which precede it, so
+// ascending order is always safe; but only ~8k rows (one wave each) are
+// resident, so on a 3D grid the planes would run nearly one after another.
+// For lof <= 1 the pivots are exactly A's entries left of the diagonal (lof
+// 0 uses none), so a caller may pass the rows ordered by their level in A's
+// lower graph instead (`order`): every pivot row lies in an earlier level,
+// and a whole level's band is in flight at once.  Fill pivots (lof >= 2)
+// are not known in advance, and those patterns keep the ascending order.
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -62,7 +71,7 @@ template <int CAP>
 __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
     int n, const int* __restrict__ arp, const int* __restrict__ aci, int lof, uint32_t* pool, long pool_cap,
     unsigned long long* pool_top, uint64_t* header, long* row_off, int* row_cnt, int* err, int* ctr,
-    uint64_t wd_limit) {
+    uint64_t wd_limit, const int* __restrict__ order) {
     __shared__ uint32_t s_buf[kSymWaves][2][CAP];
     __shared__ uint32_t s_new[kSymWaves][64];
     const int ticket = take_ticket(ctr);
@@ -71,7 +80,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
     const int wave = threadIdx.x >> 6;
     const long ri = static_cast<long>(ticket) * kSymWaves + wave;
     if (ri < n) {
-        const int i = static_cast<int>(ri);
+        const int i = order ? order[ri] : static_cast<int>(ri);
         uint32_t* cur = s_buf[wave][0];
         uint32_t* nxt = s_buf[wave][1];
         uint32_t* fresh = s_new[wave];
@@ -278,7 +287,8 @@ void check(hipError_t e, const char* what) {
 }  // namespace
 
 int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_aval, int lof, SymbolicResult& out,
-                    hipStream_t s, double watchdog_s) {
+                    hipStream_t s, double watchdog_s, const int* order) {
+    if (order && lof > 1) throw std::runtime_error("IlukSymbolic: a row order applies to lof <= 1 only");
     if (n <= 0) return -1;
     if (n >= (1 << 27) - 1) throw std::runtime_error("IlukSymbolic: n must be below 2^27 - 1");
     if (lof < 0 || lof > 31) throw std::runtime_error("IlukSymbolic: level of fill must be in [0, 31]");
@@ -308,11 +318,11 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
         if (short_rows)
             k_symbolic<kSymCapShort><<<grid, kSymWaves * 64, 0, s>>>(
                 n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
-                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s));
+                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s), order);
         else
             k_symbolic<kSymCapLong><<<grid, kSymWaves * 64, 0, s>>>(
                 n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
-                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s));
+                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s), order);
         check(hipGetLastError(), "k_symbolic");
         int err = 0;
         check(hipMemcpyAsync(&err, misc.get() + 2, sizeof(int), hipMemcpyDeviceToHost, s), "err");

@@ -47,6 +47,33 @@ def test_matches_oracle_random(oracle, lof, shape):
     assert np.array_equal(fva, o[3]) and np.array_equal(nz, o[4])
 
 
+@pytest.mark.parametrize("lof", [0, 1])
+@pytest.mark.parametrize("case", ["random", "convdiff", "poisson", "unsorted"])
+def test_level_ordered_pass_matches_oracle(oracle, case, lof):
+    """From n = 4096 with lof <= 1 the pass takes the rows by level of A's
+    lower graph (every pivot row in an earlier level); with a row of A out
+    of column order it keeps the ascending order.  Either way: bit-exact."""
+    if case == "random":
+        rp, ci = random_pattern(6000, 6, 300, seed=11)
+    elif case == "convdiff":
+        a = gen.convdiff3d(18)
+        rp, ci = a.row_ptr, a.col_idx
+    elif case == "poisson":
+        a = gen.poisson2d(80)
+        rp, ci = a.row_ptr, a.col_idx
+    else:
+        rp, ci = random_pattern(5000, 5, 200, seed=12)
+        ci = ci.copy()
+        for i in range(0, 5000, 7):  # every 7th row reversed
+            ci[rp[i]:rp[i + 1]] = ci[rp[i]:rp[i + 1]][::-1]
+    va = dominant_values(rp, ci, seed=lof)
+    frp, fci, fva, flev, nz = gpu_symbolic(rp, ci, va, lof)
+    o = oracle.iluk_symbolic(rp, ci, va, lof)
+    assert np.array_equal(frp, o[0]) and np.array_equal(fci, o[1])
+    assert np.array_equal(flev.astype(np.int32), o[2])
+    assert np.array_equal(fva, o[3]) and np.array_equal(nz, o[4])
+
+
 def test_unsorted_rows_follow_reference_value_placement(oracle):
     # A's row given out of order: the reference sorts the pattern (std::map)
     # but hands out A's values in A's entry order to the level-0 slots.
