@@ -73,6 +73,26 @@ public:
     // up, the kernel drains and the call throws; a solver whose analysis or
     // factorization timed out refuses further sweeps until it is redone.
     void SetWatchdog(double seconds);
+    // Pivot check of the following factorizations, as MKL's dcsrilu0
+    // documents it (the reference's USE_MKL_ILU build sets ipar[30] = 0):
+    // policy 0 none (the default build's behaviour); 1 a pivot that is zero
+    // or below `tiny` in magnitude fails Factorize (ipar[30] = 0, dpar[30]
+    // = tiny); 2 such a pivot is replaced by `replacement` and the
+    // factorization goes on (ipar[30] != 0, dpar[31] = replacement).
+    // reference: src/ilu_solver.cpp:248-259,631-639
+    void SetPivotPolicy(int policy, double tiny, double replacement);
+    // dcsrilu0-style code of the last factorization under policy 1: 0, -102
+    // (the first failing pivot, by row, is zero) or -103 (it is below tiny).
+    int FactorStatus() const;
+    // One triangular solve with the factors, as mkl_sparse_d_trsv
+    // (non-transposed; the reference's USE_MKL_SV build applies the
+    // preconditioner as two of them): upper false, y = alpha L^-1 x with L
+    // the unit lower triangle; upper true, y = alpha U^-1 x with U the upper
+    // triangle and its diagonal.  x and y are the same vector or do not
+    // overlap.  reference: src/ilu_solver.cpp:753-773
+    void TrsvDevice(bool upper, double alpha, const double* d_x, double* d_y);
+    // The same with host vectors.
+    void Trsv(bool upper, double alpha, const double* x, double* y);
     // Per-row diagonal compensation used by the next factorizations:
     // d_omega (device, n floats, caller-owned; null turns it off).  Row i
     // adds omega[i] times the sum of its dropped fill updates to its pivot.

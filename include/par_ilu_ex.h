@@ -33,6 +33,36 @@ int IluSolverSetSweepLanesEx(IluSolverHdl hdl, int lanes_l, int lanes_u);
 #define IluSolverSetDiagonalCompensationEx LLJBASH_DECL(IluSolverSetDiagonalCompensationEx)
 int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega);
 
+/* Pivot check of the following factorizations, as MKL's dcsrilu0
+ * documents it (replaces the reference's USE_MKL_ILU factorization,
+ * src/ilu_solver.cpp:248-259,631-639, which calls dcsrilu0 with
+ * ipar[30] = 0): policy 0 none (default); 1 a pivot that is zero or below
+ * `tiny` in magnitude makes Factorize fail (dpar[30] = tiny, 1e-16 in
+ * dcsrilu0); 2 such a pivot is replaced by `replacement` (ipar[30] != 0,
+ * dpar[31] = replacement, 1e-10 in dcsrilu0) and the factorization goes on.
+ * Factors after a policy-1 failure are unspecified. */
+#define IluSolverSetPivotPolicyEx LLJBASH_DECL(IluSolverSetPivotPolicyEx)
+int IluSolverSetPivotPolicyEx(IluSolverHdl hdl, int policy, double tiny, double replacement);
+
+/* dcsrilu0's ierr for the last factorization under policy 1: 0, -102 (the
+ * first failing pivot, by row, is zero) or -103 (it is below tiny); -1 on
+ * a bad handle.  (-101 and -106, a missing diagonal and unsorted columns,
+ * are refused earlier by IluSolverSetup with -2 and -1.) */
+#define IluSolverFactorStatusEx LLJBASH_DECL(IluSolverFactorStatusEx)
+int IluSolverFactorStatusEx(IluSolverHdl hdl);
+
+/* One triangular solve with the factors, replacing the reference's
+ * mkl_sparse_d_trsv calls (USE_MKL_SV, src/ilu_solver.cpp:753-773):
+ * upper 0, y = alpha L^-1 x with L the unit lower triangle; upper 1,
+ * y = alpha U^-1 x with U the upper triangle and its diagonal.  Device
+ * vectors of n doubles; x and y are the same vector or do not overlap. */
+#define IluSolverTrsvDeviceEx LLJBASH_DECL(IluSolverTrsvDeviceEx)
+int IluSolverTrsvDeviceEx(IluSolverHdl hdl, int upper, double alpha, const double* d_x, double* d_y);
+
+/* The same with host vectors (copied to and from the device). */
+#define IluSolverTrsvEx LLJBASH_DECL(IluSolverTrsvEx)
+int IluSolverTrsvEx(IluSolverHdl hdl, int upper, double alpha, const double* x, double* y);
+
 /* Wall-clock limit in seconds (0..3600, default 10) of every bounded
  * dependency wait in the analysis, factorization and sweep kernels; the
  * symbolic pass waits up to twice as long.  A wait past it makes the call

@@ -186,6 +186,48 @@ int orc_ilu_factorize_comp(int n, const int* rp, const int* ci, const int* diag,
 }
 
 /* ------------------------------------------------------------------ */
+/* The factorization of the reference's USE_MKL_ILU build, which calls   */
+/* MKL's dcsrilu0 (src/ilu_solver.cpp:248-259,631-639; MKL is closed and */
+/* absent, so this restates its documented contract, not its code): ILU  */
+/* on the given pattern -- the same IKJ elimination as                   */
+/* orc_ilu_factorize -- with each row's final pivot checked before later */
+/* rows use it.  policy 1 (ipar[30] = 0): a pivot that is zero or below  */
+/* tiny (dpar[30]) in magnitude stops the factorization with -102 (zero) */
+/* or -103 (small); policy 2 (ipar[30] != 0): it is replaced by repl      */
+/* (dpar[31]) and the factorization goes on; policy 0: no check.  Also   */
+/* dcsrilu0's input checks: -101 a row without its diagonal, -106 a row  */
+/* whose columns are not ascending.  Returns 0 or that code.             */
+/* ------------------------------------------------------------------ */
+int orc_ilu0_guarded(int n, const int* rp, const int* ci, const int* diag, double* val,
+                     int policy, double tiny, double repl) {
+    for (int i = 0; i < n; ++i) {
+        if (diag[i] >= rp[i + 1] || ci[diag[i]] != i) return -101;
+        for (int p = rp[i]; p + 1 < rp[i + 1]; ++p)
+            if (ci[p] >= ci[p + 1]) return -106;
+    }
+    double* w = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    if (!w) return -1;
+    int status = 0;
+    for (int i = 0; i < n && status == 0; ++i) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p) w[ci[p]] = val[p];
+        for (int p = rp[i]; p < diag[i]; ++p) {
+            int k = ci[p];
+            w[k] = w[k] / val[diag[k]];
+            double lik = w[k];
+            for (int q = diag[k] + 1; q < rp[k + 1]; ++q) w[ci[q]] = fma(-val[q], lik, w[ci[q]]);
+        }
+        double d = w[i];
+        if (policy != 0 && (d == 0.0 || fabs(d) < tiny)) {
+            if (policy == 1) status = d == 0.0 ? -102 : -103;
+            else w[i] = repl;
+        }
+        for (int p = rp[i]; p < rp[i + 1]; ++p) val[p] = w[ci[p]];
+    }
+    free(w);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
 /* Triangular sweeps, src/ilu_solver.cpp:775-796 (one-thread path).    */
 /* ------------------------------------------------------------------ */
 void orc_substitute(int n, const int* rp, const int* ci, const int* diag,
@@ -229,6 +271,26 @@ void orc_substitute_lanes(int n, const int* rp, const int* ci, const int* diag, 
     if (b != x) memcpy(x, b, (size_t)n * sizeof(double));
     for (int i = 0; i < n; ++i) x[i] = lane_sum(g, x[i], rp[i], diag[i], ci, val, x);
     for (int i = n - 1; i >= 0; --i) x[i] = lane_sum(g, x[i], diag[i] + 1, rp[i + 1], ci, val, x) / val[diag[i]];
+}
+
+/* ------------------------------------------------------------------ */
+/* One triangle of orc_substitute_lanes, as the reference's USE_MKL_SV  */
+/* build applies the preconditioner (two mkl_sparse_d_trsv calls,        */
+/* src/ilu_solver.cpp:753-773): upper 0, y = alpha L^-1 x with L the     */
+/* unit lower triangle; upper 1, y = alpha U^-1 x with U the upper       */
+/* triangle and its diagonal.  Row sums in the g-lane order; alpha is    */
+/* applied to the finished solution.  x may alias y.                     */
+/* ------------------------------------------------------------------ */
+void orc_trsv_lanes(int n, const int* rp, const int* ci, const int* diag, const double* val,
+                    int upper, double alpha, const double* x, double* y, int g) {
+    if (x != y) memcpy(y, x, (size_t)n * sizeof(double));
+    if (!upper) {
+        for (int i = 0; i < n; ++i) y[i] = lane_sum(g, y[i], rp[i], diag[i], ci, val, y);
+    } else {
+        for (int i = n - 1; i >= 0; --i) y[i] = lane_sum(g, y[i], diag[i] + 1, rp[i + 1], ci, val, y) / val[diag[i]];
+    }
+    if (alpha != 1.0)
+        for (int i = 0; i < n; ++i) y[i] *= alpha;
 }
 
 /* ------------------------------------------------------------------ */

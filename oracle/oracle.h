@@ -21,6 +21,13 @@
  *  orc_ilu_factorize  - (L U)_ij = A_ij on pattern(F); tridiagonal closed
  *                       form; exact dense LU (numpy) when lof >= bandwidth.
  *  orc_substitute     - dense triangular solves (numpy) and L(Ux) = b.
+ *  orc_ilu0_guarded   - policy 0 equals orc_ilu_factorize; hand-worked
+ *                       3x3/4x4 closed forms for the stop codes (-102,
+ *                       -103, first failing row wins), the replaced pivot
+ *                       and its effect on later rows, and -101/-106.
+ *  orc_trsv_lanes     - L^-1 then U^-1 equals orc_substitute_lanes bit for
+ *                       bit; L y = x and U y = x (scipy triangular solve);
+ *                       alpha scaling exact.
  *  orc_spmv           - exact math.fsum dot products.
  *  orc_gmres          - invariants: monotone residual estimate within a
  *                       cycle, convergence in <= n steps with M = I and
@@ -72,6 +79,21 @@ void orc_substitute(int n, const int* rp, const int* ci, const int* diag,
  * orc_substitute. */
 void orc_substitute_lanes(int n, const int* rp, const int* ci, const int* diag, const double* val,
                           const double* b, double* x, int g);
+
+/* The factorization of the reference's USE_MKL_ILU build (dcsrilu0,
+ * src/ilu_solver.cpp:248-259,631-639), restated from MKL's documented
+ * contract: orc_ilu_factorize's elimination with each row's final pivot
+ * checked -- policy 1 stops with -102 (zero) or -103 (|u_ii| < tiny),
+ * policy 2 replaces it by repl, policy 0 does not check; -101 a row
+ * without its diagonal, -106 columns not ascending.  Returns 0 or a code. */
+int orc_ilu0_guarded(int n, const int* rp, const int* ci, const int* diag, double* val,
+                     int policy, double tiny, double repl);
+
+/* One triangle of orc_substitute_lanes, as the USE_MKL_SV build's
+ * mkl_sparse_d_trsv calls (src/ilu_solver.cpp:753-773): upper 0,
+ * y = alpha L^-1 x (unit lower); upper 1, y = alpha U^-1 x.  x may alias y. */
+void orc_trsv_lanes(int n, const int* rp, const int* ci, const int* diag, const double* val,
+                    int upper, double alpha, const double* x, double* y, int g);
 
 /* y = alpha A x + beta y, row sums accumulated in entry order. */
 void orc_spmv(int n, const int* rp, const int* ci, const double* val,

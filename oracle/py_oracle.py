@@ -36,6 +36,9 @@ def _load(fast: bool = False) -> C.CDLL:
     lib.orc_ilu_factorize_comp.restype = C.c_int
     lib.orc_ilu_factorize_comp.argtypes = [C.c_int, _I, _I, _I, _D, C.POINTER(C.c_float)]
     lib.orc_substitute.argtypes = [C.c_int, _I, _I, _I, _D, _D, _D]
+    lib.orc_ilu0_guarded.restype = C.c_int
+    lib.orc_ilu0_guarded.argtypes = [C.c_int, _I, _I, _I, _D, C.c_int, C.c_double, C.c_double]
+    lib.orc_trsv_lanes.argtypes = [C.c_int, _I, _I, _I, _D, C.c_int, C.c_double, _D, _D, C.c_int]
     lib.orc_substitute_lanes.argtypes = [C.c_int, _I, _I, _I, _D, _D, _D, C.c_int]
     lib.orc_spmv.argtypes = [C.c_int, _I, _I, _D, C.c_double, _D, C.c_double, _D]
     lib.orc_gmres.restype = C.c_int
@@ -115,6 +118,29 @@ def ilu_factorize_comp(row_ptr, col_idx, value, omega, fast: bool = False):
     om = np.ascontiguousarray(omega, dtype=np.float32)
     lib.orc_ilu_factorize_comp(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(val), om.ctypes.data_as(C.POINTER(C.c_float)))
     return val, dg
+
+
+def ilu0_guarded(row_ptr, col_idx, value, policy: int, tiny: float = 1e-16, repl: float = 1e-10):
+    """dcsrilu0-contract factorization (orc_ilu0_guarded) -> (factors, diag, status)."""
+    lib = _load()
+    rp, ci = _i(row_ptr), _i(col_idx)
+    val = _d(value).copy()
+    dg = diag_ptr(rp, ci)
+    st = lib.orc_ilu0_guarded(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(val), int(policy), float(tiny),
+                              float(repl))
+    return val, dg, st
+
+
+def trsv_lanes(row_ptr, col_idx, diag, factors, x, upper: bool, alpha: float = 1.0, lanes: int = 1):
+    """y = alpha L^-1 x (upper False, L unit lower) or alpha U^-1 x (upper
+    True) with row sums in the g-lane order of the GPU sweeps."""
+    lib = _load()
+    rp, ci, dg, fv = _i(row_ptr), _i(col_idx), _i(diag), _d(factors)
+    xx = _d(x)
+    y = np.empty_like(xx)
+    lib.orc_trsv_lanes(rp.size - 1, _pi(rp), _pi(ci), _pi(dg), _pd(fv), int(bool(upper)), float(alpha), _pd(xx),
+                       _pd(y), int(lanes))
+    return y
 
 
 def substitute(row_ptr, col_idx, diag, factors, b, fast: bool = False):

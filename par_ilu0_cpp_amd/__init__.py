@@ -230,6 +230,29 @@ class IluSolver:
         if fn("IluSolverSetDiagonalCompensationEx")(self.hdl, C.c_void_p(d_omega or None)) != 0:
             raise RuntimeError("SetDiagonalCompensation failed")
 
+    def SetPivotPolicy(self, policy: int, tiny: float = 1e-16, replacement: float = 1e-10) -> None:
+        """dcsrilu0-style pivot check of the next factorizations: 0 none,
+        1 fail on a pivot below `tiny`, 2 replace it by `replacement`."""
+        if fn("IluSolverSetPivotPolicyEx")(self.hdl, int(policy), float(tiny), float(replacement)) != 0:
+            raise RuntimeError("SetPivotPolicy failed")
+
+    def FactorStatus(self) -> int:
+        return int(fn("IluSolverFactorStatusEx")(self.hdl))
+
+    def TrsvDevice(self, upper: bool, alpha: float, d_x: int, d_y: int) -> None:
+        if fn("IluSolverTrsvDeviceEx")(self.hdl, int(bool(upper)), float(alpha), C.c_void_p(d_x),
+                                       C.c_void_p(d_y)) != 0:
+            raise RuntimeError("IluSolverTrsvDeviceEx failed")
+
+    def Trsv(self, upper: bool, alpha: float, x: np.ndarray) -> np.ndarray:
+        """y = alpha T^-1 x for T = L (unit lower) or U (host vectors)."""
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(xx)
+        if fn("IluSolverTrsvEx")(self.hdl, int(bool(upper)), float(alpha), _ptr(xx, C.c_double),
+                                 _ptr(y, C.c_double)) != 0:
+            raise RuntimeError("IluSolverTrsvEx failed")
+        return y
+
     def SetWatchdog(self, seconds: float) -> None:
         """Wall-clock limit of every bounded dependency wait (0..3600 s)."""
         if fn("IluSolverSetWatchdogEx")(self.hdl, float(seconds)) != 0:

@@ -87,6 +87,10 @@ SIGNATURES = [
     ("IluSolverGetLevelsEx", _int, [_vp, P(_int), P(_int)]),
     ("IluSolverScheduleJsonEx", _int, [_vp, C.c_char_p, _int]),
     ("IluSolverSetWatchdogEx", _int, [_vp, _dbl]),
+    ("IluSolverSetPivotPolicyEx", _int, [_vp, _int, _dbl, _dbl]),
+    ("IluSolverFactorStatusEx", _int, [_vp]),
+    ("IluSolverTrsvDeviceEx", _int, [_vp, _int, _dbl, _vp, _vp]),
+    ("IluSolverTrsvEx", _int, [_vp, _int, _dbl, P(_dbl), P(_dbl)]),
     ("IluSolverAutotuneEx", _int, [_vp, _int, P(_dbl), _int]),
     ("IluSolverSetSweepModeEx", _int, [_vp, _int, _int, _int]),
     ("IluSolverSetGridShapeEx", _int, [_vp, _int, _int, _int]),

@@ -44,6 +44,22 @@ int IluSolverSetDiagonalCompensationEx(IluSolverHdl hdl, const float* d_omega) {
     return guarded("SetDiagonalCompensation", [&] { S(hdl)->SetDiagonalCompensation(d_omega); return 0; });
 }
 
+int IluSolverSetPivotPolicyEx(IluSolverHdl hdl, int policy, double tiny, double replacement) {
+    return guarded("SetPivotPolicy", [&] { S(hdl)->SetPivotPolicy(policy, tiny, replacement); return 0; });
+}
+
+int IluSolverFactorStatusEx(IluSolverHdl hdl) {
+    return guarded("FactorStatus", [&] { return S(hdl)->FactorStatus(); });
+}
+
+int IluSolverTrsvDeviceEx(IluSolverHdl hdl, int upper, double alpha, const double* d_x, double* d_y) {
+    return guarded("TrsvDevice", [&] { S(hdl)->TrsvDevice(upper != 0, alpha, d_x, d_y); return 0; });
+}
+
+int IluSolverTrsvEx(IluSolverHdl hdl, int upper, double alpha, const double* x, double* y) {
+    return guarded("Trsv", [&] { S(hdl)->Trsv(upper != 0, alpha, x, y); return 0; });
+}
+
 int IluSolverSetWatchdogEx(IluSolverHdl hdl, double seconds) {
     return guarded("SetWatchdog", [&] { S(hdl)->SetWatchdog(seconds); return 0; });
 }

@@ -113,6 +113,11 @@ struct lljbash::IluSolver::Ext {
     pilu::DevBuf<signed char> fill_level;
     long nnz_a = 0;
     const float* omega_ptr = nullptr;  // per-row compensation (device, caller-owned)
+    // Pivot check of the factorization (PivotGuard): 0 none, 1 stop, 2
+    // replace; the dcsrilu0 defaults dpar[30] = 1e-16, dpar[31] = 1e-10.
+    int pivot_mode = 0;
+    double pivot_tiny = 1e-16, pivot_repl = 1e-10;
+    int factor_status = 0;  // dcsrilu0-style code of the last factorization
 
     int precision = 64;
     // Wall-clock limit of every bounded dependency wait of the analysis,

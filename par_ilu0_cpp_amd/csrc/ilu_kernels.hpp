@@ -969,6 +969,35 @@ __global__ void k_pack_gather(long ne, const int* __restrict__ src, const TI* __
             pdiag[k] = static_cast<TO>(fac[dg[rowinfo[k].x]]);
 }
 
+// Check of every row's final pivot u_ii before the row is published, as
+// MKL's dcsrilu0 documents it for its ipar[30], dpar[30] and dpar[31] (the
+// reference's USE_MKL_ILU build, src/ilu_solver.cpp:248-259,631-639):
+// mode 0 no check (the reference's default build); mode 1 (ipar[30] = 0)
+// a pivot that is zero or smaller in magnitude than `tiny` fails the
+// factorization, and the smallest such row is recorded in *first as 2 row
+// (zero) or 2 row + 1 (small); mode 2 (ipar[30] != 0) such a pivot is
+// replaced by `repl` and the factorization continues.
+struct PivotGuard {
+    int mode;
+    double tiny;
+    double repl;
+    int* first;
+};
+
+// Applies the guard to row `row`'s pivot in LDS (lane t == 0 of the group).
+__device__ __forceinline__ void guard_pivot(const PivotGuard& pg, double* piv, int row, int t) {
+    if (pg.mode != 0 && t == 0) {
+        const double d = *piv;
+        if (d == 0.0 || fabs(d) < pg.tiny) {
+            if (pg.mode == 1) atomicMin(pg.first, 2 * row + (d == 0.0 ? 0 : 1));
+            else *piv = pg.repl;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // ---------------------------------------------------------------------
 // Numeric ILU, row IKJ form of left_looking_col (src/ilu_solver.cpp:553-589):
 //   for pivots k of row i in ascending column order:
@@ -985,7 +1014,8 @@ __global__ void k_pack_gather(long ne, const int* __restrict__ src, const TI* __
 // omega (optional): per-row diagonal compensation.  Updates u_kj * a_ik that
 // fall outside row i's pattern are summed (lane-strided, then a fixed
 // butterfly) and omega[i] times that sum is added to a_ii before the row is
-// published -- modified ILU with a per-row relaxation factor.
+// published -- modified ILU with a per-row relaxation factor.  pg: the
+// pivot check above, after the compensation.
 // ---------------------------------------------------------------------
 template <int G, int CAP>
 __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict__ perm,
@@ -994,7 +1024,7 @@ __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict_
                                                    const int* __restrict__ dg,
                                                    const double* __restrict__ vin, double* vout,
                                                    float* f32, int* err, int* ctr, uint64_t wd_limit,
-                                                   const float* __restrict__ omega) {
+                                                   const float* __restrict__ omega, const PivotGuard pg) {
     constexpr int R = 64 / G;
     constexpr int GROUPS = kBlock / G;
     __shared__ int s_col[GROUPS][CAP];
@@ -1086,6 +1116,7 @@ __global__ __launch_bounds__(kBlock) void k_factor(int n, const int* __restrict_
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 }
+                if (pg.mode != 0) guard_pivot(pg, val + ndiag, row, t);
                 for (int q = t; q < len; q += G) {
                     const double v = val[q];
                     if (f32) f32[rb + q] = static_cast<float>(v);
@@ -1117,7 +1148,7 @@ __global__ __launch_bounds__(kBlock) void k_factor_short(int n, const int* __res
                                                          const int* __restrict__ dg,
                                                          const double* __restrict__ vin, double* vout,
                                                          float* f32, int* err, int* ctr, uint64_t wd_limit,
-                                                         const float* __restrict__ omega) {
+                                                         const float* __restrict__ omega, const PivotGuard pg) {
     constexpr int R = 64 / G;
     constexpr int GROUPS = kBlock / G;
     __shared__ int s_col[GROUPS][CAP];
@@ -1223,6 +1254,7 @@ __global__ __launch_bounds__(kBlock) void k_factor_short(int n, const int* __res
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 }
+                if (pg.mode != 0) guard_pivot(pg, val + ndiag, row, t);
                 for (int q = t; q < len; q += G) {
                     const double v = val[q];
                     if (f32) f32[rb + q] = static_cast<float>(v);

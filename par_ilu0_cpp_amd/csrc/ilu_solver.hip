@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -54,7 +56,7 @@ uint64_t watchdog_ticks(double seconds) {
 
 // Error-word slots of IluSolver::Ext::err.
 enum ErrSlot { kErrFactor = 0, kErrSweep = 1, kErrSorted = 2, kErrDiag = 3, kErrMaxLev = 4, kErrRowPtr = 5,
-               kErrLevels = 6 };
+               kErrLevels = 6, kErrPivotRow = 7 };
 
 void check_sweep_errors(lljbash::IluSolver::Ext* e) {
     int v = 0;
@@ -1562,6 +1564,10 @@ int effective_lanes(const Ext* e, bool upper) {
     return lanes_for_avg(tri);
 }
 
+PivotGuard pivot_guard(Ext* e) {
+    return PivotGuard{e->pivot_mode, e->pivot_tiny, e->pivot_repl, e->err.get() + kErrPivotRow};
+}
+
 template <int G, int CAP, int P>
 void factor_short_one(Ext* e, float* f32) {
     constexpr int groups = kBlock / G;
@@ -1569,7 +1575,8 @@ void factor_short_one(Ext* e, float* f32) {
     k_factor_short<G, CAP, P><<<grid, kBlock, 0, e->stream>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
                                                              e->dg.get(), e->vin.get(), e->fac.get(), f32,
                                                              e->err.get() + kErrFactor, e->ctr.get() + 2,
-                                                             watchdog_ticks(e->watchdog_s), e->omega_ptr);
+                                                             watchdog_ticks(e->watchdog_s), e->omega_ptr,
+                                                             pivot_guard(e));
 }
 
 template <int G, int CAP>
@@ -1579,7 +1586,7 @@ void factor_one(Ext* e, float* f32) {
     k_factor<G, CAP><<<grid, kBlock, 0, e->stream>>>(e->n, e->perm_l.get(), e->rp.get(), e->ci.get(),
                                                     e->dg.get(), e->vin.get(), e->fac.get(), f32,
                                                     e->err.get() + kErrFactor, e->ctr.get() + 2,
-                                                    watchdog_ticks(e->watchdog_s), e->omega_ptr);
+                                                    watchdog_ticks(e->watchdog_s), e->omega_ptr, pivot_guard(e));
 }
 
 // Runs the factorization kernel on e->vin; returns false on a zero pivot.
@@ -1594,6 +1601,8 @@ bool run_factor(Ext* e) {
         f32 = e->fac32.get();
     }
     PILU_HIP(hipMemsetAsync(e->err.get() + kErrFactor, 0, sizeof(int), s));
+    if (e->pivot_mode == 1) k_fill_int<<<1, 64, 0, s>>>(1, e->err.get() + kErrPivotRow, INT_MAX);
+    e->factor_status = 0;
     const int mr = e->max_row;
     // short rows (stencils up to ILU(1) of a 3D 7-point operator): every
     // pivot row fetched at once, one entry per lane
@@ -1607,13 +1616,17 @@ bool run_factor(Ext* e) {
     const int err = read_int(e->err.get() + kErrFactor, s);
     if (err & kErrTimeout) throw std::runtime_error("device factorization: dependency wait timed out (watchdog)");
     if (err & 2) throw std::runtime_error("device factorization: row exceeded the LDS staging capacity");
+    if (e->pivot_mode == 1) {
+        const int first = read_int(e->err.get() + kErrPivotRow, s);
+        if (first != INT_MAX) e->factor_status = (first & 1) ? -103 : -102;
+    }
     e->factored = true;
     ++e->factor_version;
     gather_pack(e, e->pack_l, false);
     gather_pack(e, e->pack_u, true);
     if (ws_current(e, e->ws_l, e->ws_l.g)) gather_ws(e, e->ws_l);
     if (ws_current(e, e->ws_u, e->ws_u.g)) gather_ws(e, e->ws_u);
-    return (err & 1) == 0;
+    return (err & 1) == 0 && e->factor_status == 0;
 }
 
 Ext* need_setup(Ext* e) {
@@ -1670,6 +1683,41 @@ void ilu_apply(Ext* e, const double* d_b, double* d_x, const int* gate) {
     arm_y(e);
     sweep_pair(e, d_b, d_x, e->y.get(), gate, e->ctr.get() + 4, nullptr);
     e->y_armed = flags_in_output(e->mode_l);
+}
+
+__global__ void k_scale_vec(long n, double alpha, double* __restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] *= alpha;
+}
+
+// y = alpha T^-1 x for one triangle of the factors: one sweep of the
+// configured kind for that direction, then the scale.  The sweep's output
+// is armed with kPending first where that kind synchronises through it; x
+// aliasing y is staged through the solver's y (which then needs re-arming).
+void ilu_trsv(Ext* e, bool upper, double alpha, const double* d_x, double* d_y) {
+    need_setup(e);
+    if (!e->factored) throw std::runtime_error("IluSolver: Factorize has not been called");
+    if (!d_x || !d_y) throw std::runtime_error("Trsv: null vector");
+    hipStream_t s = e->stream;
+    const double* in = d_x;
+    if (d_x == d_y) {
+        PILU_HIP(hipMemcpyAsync(e->y.get(), d_x, sizeof(double) * e->n, hipMemcpyDeviceToDevice, s));
+        in = e->y.get();
+        e->y_armed = false;
+    }
+    if (flags_in_output(upper ? e->mode_u : e->mode_l))
+        k_fill_u64<<<grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, reinterpret_cast<uint64_t*>(d_y), kPending);
+    const int g = effective_lanes(e, upper);
+    int* ctr = e->ctr.get() + (upper ? 6 : 4);
+    if (upper) {
+        if (e->precision == 32) sweep_dispatch<float, true>(g, e, e->fac32.get(), in, d_y, nullptr, nullptr, ctr);
+        else sweep_dispatch<double, true>(g, e, e->fac.get(), in, d_y, nullptr, nullptr, ctr);
+    } else {
+        if (e->precision == 32) sweep_dispatch<float, false>(g, e, e->fac32.get(), in, d_y, nullptr, nullptr, ctr);
+        else sweep_dispatch<double, false>(g, e, e->fac.get(), in, d_y, nullptr, nullptr, ctr);
+    }
+    if (alpha != 1.0) k_scale_vec<<<grid_for(e->n, 256 * 8), 256, 0, s>>>(e->n, alpha, d_y);
+    PILU_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------
@@ -1878,6 +1926,31 @@ void IluSolver::SubstituteDevice(const double* d_b, double* d_x) {
 
 void IluSolver::SubstituteDeviceNoAlloc(const double* d_b, double* d_x) {
     pilu::ilu_apply_no_alloc(pilu::need_setup(ext_), d_b, d_x);
+}
+
+void IluSolver::SetPivotPolicy(int policy, double tiny, double replacement) {
+    if (policy < 0 || policy > 2) throw std::runtime_error("pivot policy must be 0 (none), 1 (stop) or 2 (replace)");
+    if (!(tiny >= 0.0) || !std::isfinite(replacement)) throw std::runtime_error("pivot policy: bad tiny or replacement");
+    if (!ext_) ext_ = new Ext;
+    ext_->pivot_mode = policy;
+    ext_->pivot_tiny = tiny;
+    ext_->pivot_repl = replacement;
+}
+
+int IluSolver::FactorStatus() const { return ext_ ? ext_->factor_status : 0; }
+
+void IluSolver::TrsvDevice(bool upper, double alpha, const double* d_x, double* d_y) {
+    pilu::ilu_trsv(pilu::need_setup(ext_), upper, alpha, d_x, d_y);
+}
+
+void IluSolver::Trsv(bool upper, double alpha, const double* x, double* y) {
+    Ext* e = pilu::need_setup(ext_);
+    hipStream_t s = e->stream;
+    e->hb.reserve(e->n);
+    PILU_HIP(hipMemcpyAsync(e->hb.get(), x, sizeof(double) * e->n, hipMemcpyHostToDevice, s));
+    pilu::ilu_trsv(e, upper, alpha, e->hb.get(), e->hb.get());
+    PILU_HIP(hipMemcpyAsync(y, e->hb.get(), sizeof(double) * e->n, hipMemcpyDeviceToHost, s));
+    pilu::check_sweep_errors(e);
 }
 
 void IluSolver::SetWatchdog(double seconds) {
