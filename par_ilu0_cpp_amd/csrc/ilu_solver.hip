@@ -178,64 +178,6 @@ void launch_levels(Ext* e, const int* block_of, int* level, int* maxlev, uint64_
     else levels_g<UPPER, 64>(e, block_of, level, maxlev, wd);
 }
 
-// End of every row's part left of the diagonal (first entry with column >=
-// the row) in a pattern whose rows are sorted; bad[0] is set for a row that
-// is not strictly increasing.
-__global__ void k_lower_end(int n, const int* __restrict__ rp, const int* __restrict__ ci, int* __restrict__ lend,
-                            int* bad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int b = rp[i], e = rp[i + 1];
-    int end = e;
-    bool ok = true;
-    for (int p = b; p < e; ++p) {
-        if (p + 1 < e && ci[p] >= ci[p + 1]) ok = false;
-        if (end == e && ci[p] >= i) end = p;
-    }
-    lend[i] = end;
-    if (!ok) atomicOr(bad, 1);
-}
-
-// The rows of A (device CSR) ordered by their level in A's lower graph
-// (k_levels on A, radix sort), for the symbolic pass with lof <= 1.  Empty
-// when a row of A is not sorted or the analysis hit its watchdog: the pass
-// then keeps the ascending order.
-DevBuf<int> lower_level_order(Ext* e, int n, int nnz_a, const int* arp, const int* aci) {
-    hipStream_t s = e->stream;
-    DevBuf<int> lend(n), lev(n), flags(8), order;
-    PILU_HIP(hipMemsetAsync(flags.get(), 0, 8 * sizeof(int), s));  // [0] bad [1] max level [2] err [4..5] tickets
-    k_lower_end<<<grid_for(n, 256), 256, 0, s>>>(n, arp, aci, lend.get(), flags.get());
-    PILU_HIP(hipGetLastError());
-    if (read_int(flags.get(), s)) return order;
-    k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, lev.get(), -1);
-    const uint64_t wd = watchdog_ticks(e->watchdog_s);
-    const double avg = n > 0 ? static_cast<double>(nnz_a - n) / (2.0 * n) : 0.0;
-    auto run = [&](auto g) {
-        constexpr int G = decltype(g)::value;
-        constexpr int rows = (kBlock / 64) * (64 / G);
-        k_levels<false, G><<<grid_for(n, rows), kBlock, 0, s>>>(n, arp, aci, lend.get(), lev.get(), flags.get() + 1,
-                                                                  flags.get() + 4, flags.get() + 2, wd, nullptr);
-    };
-    if (avg <= 8) run(std::integral_constant<int, 1>{});
-    else if (avg <= 32) run(std::integral_constant<int, 8>{});
-    else run(std::integral_constant<int, 64>{});
-    PILU_HIP(hipGetLastError());
-    if (read_int(flags.get() + 2, s) & kErrTimeout) return order;
-    const int nlev = read_int(flags.get() + 1, s) + 1;
-    DevBuf<int> keys_out(n), rows(n);
-    k_iota<<<grid_for(n, 256), 256, 0, s>>>(n, rows.get());
-    order.resize(n);
-    unsigned end_bit = 1;
-    while (end_bit < 31 && (1u << end_bit) <= static_cast<unsigned>(nlev)) ++end_bit;
-    size_t bytes = 0;
-    PILU_HIP(rocprim::radix_sort_pairs(nullptr, bytes, lev.get(), keys_out.get(), rows.get(), order.get(),
-                                       static_cast<unsigned>(n), 0u, end_bit, s));
-    e->sort_tmp.reserve(bytes);
-    PILU_HIP(rocprim::radix_sort_pairs(e->sort_tmp.get(), bytes, lev.get(), keys_out.get(), rows.get(), order.get(),
-                                       static_cast<unsigned>(n), 0u, end_bit, s));
-    return order;
-}
-
 // Levels of one dependency graph and the rows sorted by (level, row).
 int build_schedule(Ext* e, bool upper, DevBuf<int>& perm, DevBuf<int>& lev_off) {
     const int n = e->n;
@@ -1867,16 +1809,8 @@ int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx,
     e->analysed = false;  // the previous pattern is gone whether or not this completes
     pilu::SymbolicResult res;
     pilu::PhaseTimer pt("symbolic", e->stream);
-    // rows by level of A's lower graph where the pivots are A's own entries
-    pilu::DevBuf<int> order;
-    if (lof <= 1 && n >= 4096) {
-        int nnz_a0 = 0;
-        PILU_HIP(hipMemcpy(&nnz_a0, d_row_ptr + n, sizeof(int), hipMemcpyDeviceToHost));
-        order = pilu::lower_level_order(e, n, nnz_a0, d_row_ptr, d_col_idx);
-        pt.mark("row order");
-    }
     const int nnz_f = pilu::symbolic_device(n, d_row_ptr, d_col_idx, nullptr, lof, res, e->stream,
-                                            2.0 * e->watchdog_s, order.get());
+                                            2.0 * e->watchdog_s);
     pt.mark("symbolic ILU(k)");
     if (nnz_f < 0) return -1;
     int nnz_a = 0;

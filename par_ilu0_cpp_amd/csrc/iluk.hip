@@ -13,14 +13,15 @@
 // exactly the words it reads.  A compaction pass then lays the rows out as
 // CSR with the reference's value placement and nzmap.
 //
-// Row order.  A row waits only for its pivot rows, which precede it, so
-// ascending order is always safe; but only ~8k rows (one wave each) are
-// resident, so on a 3D grid the planes would run nearly one after another.
-// For lof <= 1 the pivots are exactly A's entries left of the diagonal (lof
-// 0 uses none), so a caller may pass the rows ordered by their level in A's
-// lower graph instead (`order`): every pivot row lies in an earlier level,
-// and a whole level's band is in flight at once.  Fill pivots (lof >= 2)
-// are not known in advance, and those patterns keep the ascending order.
+// Rows that need no other row.  For lof <= 1 a pivot k of row i can only
+// be one of A's own entries (level 0), and only level-0 entries of row k
+// can give fill of level <= 1 -- exactly A's entries right of k's
+// diagonal.  So when every row of A is sorted the pass reads pivot rows
+// from A itself (`a_pivots`): no row waits for another, and all rows run at
+// full occupancy instead of a window of ~8k ascending rows (on a 3D grid
+// the planes would otherwise run nearly one after another).  The result is
+// the same pattern, levels and order as the waiting form; lof >= 2, and A
+// with a row out of order or without its diagonal, take the waiting form.
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -37,6 +38,7 @@ namespace pilu {
 namespace {
 
 constexpr int kSymWaves = 4;           // waves (rows) per workgroup
+constexpr int kSymGridA = 4096;        // workgroups of the form that reads pivot rows from A
 // LDS entries per row buffer: a short-row form (more rows resident, so a
 // longer stretch of the row order is in flight) and the general form, used
 // when a row of F outgrows the short one.
@@ -71,16 +73,20 @@ template <int CAP>
 __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
     int n, const int* __restrict__ arp, const int* __restrict__ aci, int lof, uint32_t* pool, long pool_cap,
     unsigned long long* pool_top, uint64_t* header, long* row_off, int* row_cnt, int* err, int* ctr,
-    uint64_t wd_limit, const int* __restrict__ order) {
+    uint64_t wd_limit, bool a_pivots) {
     __shared__ uint32_t s_buf[kSymWaves][2][CAP];
     __shared__ uint32_t s_new[kSymWaves][64];
-    const int ticket = take_ticket(ctr);
+    // waiting form: rows in ticket order, one per wave; rows from A: a
+    // fixed grid of waves striding over the rows (no order needed)
+    const int ticket = a_pivots ? 0 : take_ticket(ctr);
     const Watchdog wd(wd_limit);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const long ri = static_cast<long>(ticket) * kSymWaves + wave;
-    if (ri < n) {
-        const int i = order ? order[ri] : static_cast<int>(ri);
+    const long first = (a_pivots ? static_cast<long>(blockIdx.x) : static_cast<long>(ticket)) * kSymWaves + wave;
+    const long stride = a_pivots ? static_cast<long>(gridDim.x) * kSymWaves : static_cast<long>(n);
+    for (long ri = first; ri < n; ri += stride) {
+        const int i = static_cast<int>(ri);
+        wave_sync_lds();  // the previous row's last reads of the buffers
         uint32_t* cur = s_buf[wave][0];
         uint32_t* nxt = s_buf[wave][1];
         uint32_t* fresh = s_new[wave];
@@ -124,19 +130,33 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
             if (k >= i) break;
             const int lk = static_cast<int>(pe & kLevMask);
             if (lk >= lof) continue;
-            uint64_t h;
-            bool late = false;
-            while ((h = peek_raw(header + k)) == kNoHeader) {
-                if (wd.expired()) { late = true; break; }
-                spin_pause();
+            long uoff = 0;
+            int ucnt = 0;
+            if (a_pivots) {
+                // A's row k (sorted): its entries right of the diagonal
+                const int kb = arp[k], ke = arp[k + 1];
+                int le = 0;
+                for (int x0 = kb; x0 < ke; x0 += 64) {
+                    const int x = x0 + lane;
+                    le += __popcll(__ballot(x < ke && aci[x] <= k));
+                }
+                uoff = kb + le;
+                ucnt = ke - kb - le;
+            } else {
+                uint64_t h;
+                bool late = false;
+                while ((h = peek_raw(header + k)) == kNoHeader) {
+                    if (wd.expired()) { late = true; break; }
+                    spin_pause();
+                }
+                if (__ballot(late) != 0) {
+                    failed = true;
+                    if (lane == 0) atomicOr(err, kErrTimeout);
+                    break;
+                }
+                uoff = static_cast<long>(h >> 24);
+                ucnt = static_cast<int>(h & 0xFFFFFFull);
             }
-            if (__ballot(late) != 0) {
-                failed = true;
-                if (lane == 0) atomicOr(err, kErrTimeout);
-                break;
-            }
-            const long uoff = static_cast<long>(h >> 24);
-            const int ucnt = static_cast<int>(h & 0xFFFFFFull);
             for (int c0 = 0; c0 < ucnt && !failed; c0 += 64) {
                 const int q = c0 + lane;
                 bool is_new = false;
@@ -145,9 +165,13 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
                 bool late = false;
                 if (q < ucnt) {
                     uint32_t e;
-                    while ((e = peek_u32(pool + uoff + q)) == kEmpty) {
-                        if (wd.expired()) { late = true; break; }
-                        spin_pause();
+                    if (a_pivots) {
+                        e = static_cast<uint32_t>(aci[uoff + q]) << kLevBits;
+                    } else {
+                        while ((e = peek_u32(pool + uoff + q)) == kEmpty) {
+                            if (wd.expired()) { late = true; break; }
+                            spin_pause();
+                        }
                     }
                     const uint32_t j = e >> kLevBits;
                     const int L = lk + static_cast<int>(e & kLevMask) + 1;
@@ -237,7 +261,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
             publish_raw(header + i, (static_cast<uint64_t>(uo) << 24) | static_cast<uint64_t>(uc));
         }
     }
-    retire_ticket(ctr);
+    if (!a_pivots) retire_ticket(ctr);
 }
 
 // Lay rows out as CSR: columns, levels, values (A's values at the level-0
@@ -280,6 +304,20 @@ __global__ __launch_bounds__(256) void k_sym_emit(int n, const int* __restrict__
     if (lane == 0 && zeros != arp[i + 1] - ab) atomicOr(err, 1);
 }
 
+// bad[0] set when a row of A is not strictly increasing or lacks its
+// diagonal (a row of F without its diagonal passes on no fill, which the
+// rows of A alone do not tell for lof = 1).
+__global__ void k_rows_sorted(int n, const int* __restrict__ rp, const int* __restrict__ ci, int* bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool ok = true, diag = false;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        if (p + 1 < rp[i + 1]) ok &= ci[p] < ci[p + 1];
+        diag |= ci[p] == i;
+    }
+    if (!ok || !diag) atomicOr(bad, 1);
+}
+
 void check(hipError_t e, const char* what) {
     if (e != hipSuccess) hip_fail(e, what, __FILE__, __LINE__);
 }
@@ -287,8 +325,7 @@ void check(hipError_t e, const char* what) {
 }  // namespace
 
 int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_aval, int lof, SymbolicResult& out,
-                    hipStream_t s, double watchdog_s, const int* order) {
-    if (order && lof > 1) throw std::runtime_error("IlukSymbolic: a row order applies to lof <= 1 only");
+                    hipStream_t s, double watchdog_s) {
     if (n <= 0) return -1;
     if (n >= (1 << 27) - 1) throw std::runtime_error("IlukSymbolic: n must be below 2^27 - 1");
     if (lof < 0 || lof > 31) throw std::runtime_error("IlukSymbolic: level of fill must be in [0, 31]");
@@ -299,7 +336,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     DevBuf<uint64_t> header(n);
     DevBuf<long> row_off(n);
     DevBuf<int> row_cnt(static_cast<size_t>(n) + 1);
-    DevBuf<int> misc(8);  // [0..1] ticket counters, [2] err
+    DevBuf<int> misc(8);  // [0..1] ticket counters, [2] err, [3] unsorted row of A
     DevBuf<unsigned long long> top(1);
     long cap = std::max<long>(static_cast<long>(nnz_a) * (lof == 0 ? 1 : 32), 1024);
     DevBuf<uint32_t> pool;
@@ -307,6 +344,18 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     // while that stays within 16 GiB); a row over its capacity retries the
     // long form
     bool short_rows = static_cast<long>(n) * kSymCapShort <= (1L << 32);
+    // lof <= 1 over sorted rows of A that hold their diagonals: pivot rows
+    // come from A (no waits)
+    bool a_pivots = false;
+    if (lof <= 1) {
+        check(hipMemsetAsync(misc.get() + 3, 0, sizeof(int), s), "sorted flag");
+        k_rows_sorted<<<ceil_div(n, 256), 256, 0, s>>>(n, d_arp, d_aci, misc.get() + 3);
+        check(hipGetLastError(), "k_rows_sorted");
+        int bad = 0;
+        check(hipMemcpyAsync(&bad, misc.get() + 3, sizeof(int), hipMemcpyDeviceToHost, s), "sorted flag");
+        check(hipStreamSynchronize(s), "sync");
+        a_pivots = bad == 0;
+    }
     for (int attempt = 0;; ++attempt) {
         if (short_rows) cap = std::max<long>(cap, static_cast<long>(n) * kSymCapShort);
         pool.resize(cap);
@@ -314,15 +363,15 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
         check(hipMemsetAsync(header.get(), 0xFF, sizeof(uint64_t) * n, s), "header fill");
         check(hipMemsetAsync(misc.get(), 0, 8 * sizeof(int), s), "misc");
         check(hipMemsetAsync(top.get(), 0, sizeof(unsigned long long), s), "top");
-        const int grid = ceil_div(n, kSymWaves);
+        const int grid = a_pivots ? std::min(ceil_div(n, kSymWaves), kSymGridA) : ceil_div(n, kSymWaves);
         if (short_rows)
             k_symbolic<kSymCapShort><<<grid, kSymWaves * 64, 0, s>>>(
                 n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
-                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s), order);
+                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s), a_pivots);
         else
             k_symbolic<kSymCapLong><<<grid, kSymWaves * 64, 0, s>>>(
                 n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
-                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s), order);
+                misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s), a_pivots);
         check(hipGetLastError(), "k_symbolic");
         int err = 0;
         check(hipMemcpyAsync(&err, misc.get() + 2, sizeof(int), hipMemcpyDeviceToHost, s), "err");

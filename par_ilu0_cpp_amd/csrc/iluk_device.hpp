@@ -19,10 +19,8 @@ struct SymbolicResult {
 // Runs the pass on device arrays of A (values optional).  Returns nnz(F),
 // or -1 for a row of A with a repeated column; throws std::runtime_error on
 // device errors, capacity limits or a dependency wait longer than
-// watchdog_s seconds.  order (optional, lof <= 1 only): the rows in an order
-// in which every row follows the rows its entries left of the diagonal
-// name, such as by level of A's lower graph; null: ascending.
+// watchdog_s seconds.
 int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_aval, int lof, SymbolicResult& out,
-                    hipStream_t s, double watchdog_s = 20.0, const int* order = nullptr);
+                    hipStream_t s, double watchdog_s = 20.0);
 
 }  // namespace pilu

@@ -48,11 +48,12 @@ def test_matches_oracle_random(oracle, lof, shape):
 
 
 @pytest.mark.parametrize("lof", [0, 1])
-@pytest.mark.parametrize("case", ["random", "convdiff", "poisson", "unsorted"])
-def test_level_ordered_pass_matches_oracle(oracle, case, lof):
-    """From n = 4096 with lof <= 1 the pass takes the rows by level of A's
-    lower graph (every pivot row in an earlier level); with a row of A out
-    of column order it keeps the ascending order.  Either way: bit-exact."""
+@pytest.mark.parametrize("case", ["random", "convdiff", "poisson", "unsorted", "nodiag", "small"])
+def test_rows_from_a_match_oracle(oracle, case, lof):
+    """With lof <= 1 and sorted rows of A the pass reads pivot rows from A
+    (no row waits for another); with a row of A out of column order or
+    without its diagonal it waits for the published rows.  Either way:
+    bit-exact."""
     if case == "random":
         rp, ci = random_pattern(6000, 6, 300, seed=11)
     elif case == "convdiff":
@@ -61,14 +62,34 @@ def test_level_ordered_pass_matches_oracle(oracle, case, lof):
     elif case == "poisson":
         a = gen.poisson2d(80)
         rp, ci = a.row_ptr, a.col_idx
+    elif case == "nodiag":  # every third row without its diagonal; lof 1 fills some in
+        rp, ci = random_pattern(5000, 5, 200, seed=13)
+        keep = np.ones(ci.size, dtype=bool)
+        for i in range(0, 5000, 3):
+            keep[rp[i] + np.nonzero(ci[rp[i]:rp[i + 1]] == i)[0]] = False
+        ci = ci[keep]
+        rp = np.concatenate([[0], np.cumsum([keep[rp[i]:rp[i + 1]].sum() for i in range(5000)])]).astype(np.int32)
+    elif case == "small":
+        rp, ci = random_pattern(300, 12, 299, seed=14)
     else:
         rp, ci = random_pattern(5000, 5, 200, seed=12)
         ci = ci.copy()
         for i in range(0, 5000, 7):  # every 7th row reversed
             ci[rp[i]:rp[i + 1]] = ci[rp[i]:rp[i + 1]][::-1]
     va = dominant_values(rp, ci, seed=lof)
-    frp, fci, fva, flev, nz = gpu_symbolic(rp, ci, va, lof)
     o = oracle.iluk_symbolic(rp, ci, va, lof)
+    if case == "nodiag":
+        # IlukSymbolic alone: the solver's analysis (which also returns the
+        # levels) refuses rows without a diagonal
+        a = P.HostCsr(rp, ci, va)
+        f = P.HostCsr()
+        nz = np.zeros(ci.size, dtype=np.int32)
+        assert P.IlukSymbolic(a, lof, f.mat, nz) == o[0][-1]
+        frp, fci, fva = f.arrays()
+        assert np.array_equal(frp, o[0]) and np.array_equal(fci, o[1])
+        assert np.array_equal(fva, o[3]) and np.array_equal(nz, o[4])
+        return
+    frp, fci, fva, flev, nz = gpu_symbolic(rp, ci, va, lof)
     assert np.array_equal(frp, o[0]) and np.array_equal(fci, o[1])
     assert np.array_equal(flev.astype(np.int32), o[2])
     assert np.array_equal(fva, o[3]) and np.array_equal(nz, o[4])
