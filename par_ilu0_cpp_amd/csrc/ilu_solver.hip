@@ -660,6 +660,26 @@ __global__ void k_pack_block_start(int nblk, const int* __restrict__ kptr, const
     for (int k = kptr[p] + threadIdx.x; k < kptr[p + 1]; k += blockDim.x) kb0[perm[k]] = kptr[p];
 }
 
+// (row, entry count in the direction) of every schedule position.
+template <bool UPPER>
+__global__ void k_pack_rowinfo(int n, const int* __restrict__ perm, const int* __restrict__ rp,
+                               const int* __restrict__ dg, int2* __restrict__ rowinfo) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int row = perm[k];
+    rowinfo[k] = make_int2(row, UPPER ? rp[row + 1] - dg[row] - 1 : dg[row] - rp[row]);
+}
+
+// Widest row of every level group.
+__global__ void k_pack_width(int ngroups, const int* __restrict__ loff, const int2* __restrict__ rowinfo,
+                             int* __restrict__ width) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    int w = 0;
+    for (int k = loff[g]; k < loff[g + 1]; ++k) w = max(w, rowinfo[k].y);
+    width[g] = w;
+}
+
 __global__ void k_pack_group_of(int ngroups, const int* __restrict__ loff, int* __restrict__ grp_of) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < ngroups)
@@ -699,13 +719,10 @@ __global__ void k_pack_fill(int n, const int* __restrict__ perm, const int* __re
 void build_pack(Ext* e, bool upper, SweepPack& pk) {
     refuse_if_no_alloc(e, "packed factors");
     const int n = e->n, nblk = e->nblk;
+    hipStream_t st = e->stream;
     const DevBuf<int>& perm = upper ? e->bperm_u : e->bperm_l;
     const DevBuf<int>& blev = upper ? e->blev_u : e->blev_l;
     const DevBuf<int>& loff = upper ? e->loff_u : e->loff_l;
-    const std::vector<int>& rp = e->h_rp;
-    const std::vector<int>& dg = e->h_dg;
-    std::vector<int> pm(n);
-    PILU_HIP(hipMemcpy(pm.data(), perm.get(), sizeof(int) * pm.size(), hipMemcpyDeviceToHost));
     pk.h_blev.assign(static_cast<size_t>(nblk) + 1, 0);
     PILU_HIP(hipMemcpy(pk.h_blev.data(), blev.get(), sizeof(int) * pk.h_blev.size(), hipMemcpyDeviceToHost));
     const int ngroups = pk.h_blev[nblk];
@@ -713,22 +730,21 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
     PILU_HIP(hipMemcpy(pk.h_loff.data(), loff.get(), sizeof(int) * pk.h_loff.size(), hipMemcpyDeviceToHost));
     std::vector<int> kptr(static_cast<size_t>(nblk) + 1);
     for (int p = 0; p <= nblk; ++p) kptr[p] = pk.h_loff[pk.h_blev[p]];
-    pk.h_base.assign(ngroups, 0);
+    // rows and widths on the device; the slab bases on the host (one pass
+    // over the groups, kept there for the pass lists)
+    pk.rowinfo.resize(2 * static_cast<size_t>(std::max(n, 1)));
+    DevBuf<int> dwidth(std::max(ngroups, 1));
+    int2* rowinfo = reinterpret_cast<int2*>(pk.rowinfo.get());
+    if (n > 0) {
+        if (upper) k_pack_rowinfo<true><<<grid_for(n, 256), 256, 0, st>>>(n, perm.get(), e->rp.get(), e->dg.get(), rowinfo);
+        else k_pack_rowinfo<false><<<grid_for(n, 256), 256, 0, st>>>(n, perm.get(), e->rp.get(), e->dg.get(), rowinfo);
+    }
+    if (ngroups > 0) k_pack_width<<<grid_for(ngroups, 256), 256, 0, st>>>(ngroups, loff.get(), rowinfo, dwidth.get());
+    PILU_HIP(hipGetLastError());
     pk.h_width.assign(ngroups, 0);
-    std::vector<int> rowinfo(2 * static_cast<size_t>(n));
-    parallel_for(ngroups, [&](long lo, long hi) {
-        for (long g = lo; g < hi; ++g) {
-            int w = 0;
-            for (int k = pk.h_loff[g]; k < pk.h_loff[g + 1]; ++k) {
-                const int row = pm[k];
-                const int cnt = upper ? rp[row + 1] - dg[row] - 1 : dg[row] - rp[row];
-                rowinfo[2 * static_cast<size_t>(k)] = row;
-                rowinfo[2 * static_cast<size_t>(k) + 1] = cnt;
-                w = std::max(w, cnt);
-            }
-            pk.h_width[g] = w;
-        }
-    });
+    if (ngroups > 0)
+        PILU_HIP(hipMemcpy(pk.h_width.data(), dwidth.get(), sizeof(int) * ngroups, hipMemcpyDeviceToHost));
+    pk.h_base.assign(ngroups, 0);
     long total = 0;
     int maxw = 0;
     for (int g = 0; g < ngroups; ++g) {
@@ -739,20 +755,16 @@ void build_pack(Ext* e, bool upper, SweepPack& pk) {
     }
     pk.ne = total;
     pk.max_w = maxw;
-    pk.rowinfo.resize(rowinfo.size());
     const size_t ne_alloc = static_cast<size_t>(std::max<long>(total, 1));
     pk.pcol.resize(ne_alloc);
     pk.psrc.resize(ne_alloc);
     pk.kptr.resize(kptr.size());
-    PILU_HIP(hipMemcpy(pk.rowinfo.get(), rowinfo.data(), sizeof(int) * rowinfo.size(), hipMemcpyHostToDevice));
     PILU_HIP(hipMemcpy(pk.kptr.get(), kptr.data(), sizeof(int) * kptr.size(), hipMemcpyHostToDevice));
     // slabs on the device (every slot is written, padding included)
     {
-        hipStream_t st = e->stream;
         DevBuf<int> kinv(std::max(n, 1)), kb0(std::max(n, 1)), grp_of(std::max(n, 1));
-        DevBuf<int> dbase(std::max(ngroups, 1)), dwidth(std::max(ngroups, 1));
+        DevBuf<int> dbase(std::max(ngroups, 1));
         PILU_HIP(hipMemcpy(dbase.get(), pk.h_base.data(), sizeof(int) * ngroups, hipMemcpyHostToDevice));
-        PILU_HIP(hipMemcpy(dwidth.get(), pk.h_width.data(), sizeof(int) * ngroups, hipMemcpyHostToDevice));
         if (total == 0) {
             const int zero = 0, minus1 = -1;
             PILU_HIP(hipMemcpy(pk.pcol.get(), &zero, sizeof(int), hipMemcpyHostToDevice));
@@ -1264,11 +1276,22 @@ int effective_lanes(const Ext* e, bool upper);
 // most 16 (records hold 64 / G rows).
 int ws_lanes(int lanes) { return std::min(std::max(lanes, 1), 16); }
 
+// Lanes per row the packed sweeps run with for a configured count
+// (sweep_pp_dispatch: 1, 2, 4, 8, else 16).
+int pp_lanes(int lanes) { return lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 ? lanes : 16; }
+
 // Builds what the configured sweep kinds need (packs, record streams), so
 // that the first apply allocates nothing more.
 void prepare_modes(Ext* e) {
-    const bool pp = (e->mode_l >= 2 && e->mode_l <= 3) || (e->mode_u >= 2 && e->mode_u <= 3);
-    if (pp) ensure_packs(e);
+    const bool pp_l = e->mode_l >= 2 && e->mode_l <= 3, pp_u = e->mode_u >= 2 && e->mode_u <= 3;
+    if (pp_l || pp_u) {
+        ensure_packs(e);
+        // pass lists for the lanes the packed sweeps will run with
+        PhaseTimer pt("packs", e->stream);
+        if (pp_l) build_passes(e, e->pack_l, pp_lanes(effective_lanes(e, false)));
+        if (pp_u) build_passes(e, e->pack_u, pp_lanes(effective_lanes(e, true)));
+        pt.mark("pass lists");
+    }
     if (e->mode_l == 5) ensure_ws(e, false, ws_lanes(effective_lanes(e, false)));
     if (e->mode_u == 5) ensure_ws(e, true, ws_lanes(effective_lanes(e, true)));
 }
