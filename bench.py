@@ -492,7 +492,44 @@ def workload_c4(count=512, nrhs=8, m=30):
                             "max_rel_res": float(rel.max()), "pair_solves_per_s": conv.size / t_sol}
         log(f"[c4] fp{bits}: {out[f'fp{bits}']}")
         del bg
+    out["cpu_baseline"] = c4_cpu_baseline(mats, sp, rhs, m)
+    log(f"[c4] cpu: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
+
+
+def c4_cpu_baseline(mats, sp, rhs, m, every=32):
+    """The reference's way through the batch on the host cores: one system at
+    a time with every thread (the oracle's OpenMP build, the reference's
+    subtree / queue schedule), ILU(0) factorized once per system, then
+    GMRES(m) per right-hand side to the same relative tolerance.  A bounded
+    sample -- every `every`-th system, all its right-hand sides -- gives
+    pair solves per second."""
+    import numpy as np
+
+    from oracle import py_oracle as O
+
+    threads = O.baseline_threads()
+    t_fac = t_gm = 0.0
+    pairs = conv = its_sum = 0
+    sample = list(range(0, len(mats), every))
+    for s in sample:
+        a = mats[s]
+        for r in range(rhs.shape[1]):
+            b = np.ascontiguousarray(rhs[sp[s]:sp[s + 1], r])
+            ret, its, x, times = O.baseline_ilu_gmres((a.row_ptr, a.col_idx, a.value),
+                                                      (a.row_ptr, a.col_idx, a.value), b,
+                                                      1e-8 * float(np.linalg.norm(b)), m, 2000, sched=1)
+            if r == 0:
+                t_fac += times[1]
+            t_gm += times[2]
+            pairs += 1
+            conv += int(ret == 1)
+            its_sum += its
+    total = t_fac + t_gm
+    return {"pair_solves_per_s": pairs / total if total > 0 else None, "pairs": pairs, "converged": conv,
+            "its_mean": its_sum / max(pairs, 1), "factorize_s": t_fac, "gmres_s": t_gm, "threads": threads,
+            "kind": "port", "sample": f"systems {sample[0]}, {sample[0] + every}, ... ({len(sample)} of "
+                                      f"{len(mats)}), all {rhs.shape[1]} right-hand sides each"}
 
 
 def traffic_bytes(path, form):
