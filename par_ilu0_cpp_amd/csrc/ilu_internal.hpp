@@ -140,9 +140,11 @@ struct lljbash::IluSolver::Ext {
     int tile[3] = {0, 0, 0}, grid[3] = {0, 0, 0};
     int skew[3] = {0, 0, 0};  // tile coordinates u = x + skew0 y + skew1 z, v = y + skew2 z
     std::vector<int> h_block_of;
-    // host copy of the pattern (row_ptr, diagonal positions, columns) for the
-    // partition and pack builders; filled once per analysed pattern
+    // host copies of the pattern (row_ptr, diagonal positions, columns) and
+    // of block_of for the wave-specialised stream builder, copied on first
+    // use (host_pattern)
     std::vector<int> h_rp, h_dg, h_ci;
+    int h_pattern_version = -1, h_part_version = -1;  // what the host copies were taken from
     pilu::DevBuf<int> block_of;
     int part_version = 0;
     long factor_version = 0;

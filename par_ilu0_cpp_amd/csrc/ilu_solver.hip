@@ -5,6 +5,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -56,7 +57,7 @@ uint64_t watchdog_ticks(double seconds) {
 
 // Error-word slots of IluSolver::Ext::err.
 enum ErrSlot { kErrFactor = 0, kErrSweep = 1, kErrSorted = 2, kErrDiag = 3, kErrMaxLev = 4, kErrRowPtr = 5,
-               kErrLevels = 6, kErrPivotRow = 7 };
+               kErrLevels = 6, kErrPivotRow = 7, kErrExtent = 8 /* 3 words */ };
 
 void check_sweep_errors(lljbash::IluSolver::Ext* e) {
     int v = 0;
@@ -122,32 +123,26 @@ void parallel_for(long n, F f) {
     for (auto& x : th) x.join();
 }
 
-// Host copy of the pattern (row_ptr and diagonal positions are kept by
-// analyse; the columns are copied on first use).
+// Host copy of the pattern, the diagonal positions and the partition,
+// copied on first use after the pattern or the partition changed (only the
+// wave-specialised sweep's stream builder needs them).
 void host_pattern(lljbash::IluSolver::Ext* e) {
-    if (static_cast<long>(e->h_ci.size()) != e->nnz) {
+    const size_t n = static_cast<size_t>(e->n);
+    if (e->h_pattern_version != e->pattern_version) {
+        e->h_rp.resize(n + 1);
+        e->h_dg.resize(n);
         e->h_ci.resize(static_cast<size_t>(e->nnz));
+        PILU_HIP(hipMemcpy(e->h_rp.data(), e->rp.get(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+        PILU_HIP(hipMemcpy(e->h_dg.data(), e->dg.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
         PILU_HIP(hipMemcpy(e->h_ci.data(), e->ci.get(), sizeof(int) * e->h_ci.size(), hipMemcpyDeviceToHost));
+        e->h_pattern_version = e->pattern_version;
+    }
+    if (e->nblk > 0 && e->h_part_version != e->part_version) {
+        e->h_block_of.resize(n);
+        PILU_HIP(hipMemcpy(e->h_block_of.data(), e->block_of.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
+        e->h_part_version = e->part_version;
     }
 }
-
-// Grid coordinates of row q: q = x + nx (y + ny z), exact for q < 2^31 (a
-// reciprocal estimate, corrected by one step).
-struct GridCoord {
-    long nx, nxy;
-    double inx, inxy;
-    GridCoord(long nx_, long ny_) : nx(nx_), nxy(nx_ * ny_), inx(1.0 / nx_), inxy(1.0 / (nx_ * ny_)) {}
-    void operator()(long q, long& x, long& y, long& z) const {
-        z = static_cast<long>(static_cast<double>(q) * inxy);
-        if (z * nxy > q) --z;
-        else if ((z + 1) * nxy <= q) ++z;
-        const long rem = q - z * nxy;
-        y = static_cast<long>(static_cast<double>(rem) * inx);
-        if (y * nx > rem) --y;
-        else if ((y + 1) * nx <= rem) ++y;
-        x = rem - y * nx;
-    }
-};
 
 using Ext = lljbash::IluSolver::Ext;
 
@@ -227,6 +222,22 @@ __global__ void k_block_keys(int n, const int* __restrict__ block_of, const int*
 // Schedule of the block-local sweeps in one direction: levels restricted to
 // each block's own rows, rows sorted by (processing position of their
 // block, local level, row), and the group offsets of every (block, level).
+// head[i]: the sorted key i starts a new (block, level) group.
+__global__ void k_group_heads(int n, const unsigned* __restrict__ keys, unsigned char* __restrict__ head) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = i == 0 || keys[i] != keys[i - 1];
+}
+
+// bl[p] = first group of block position p (the block part of the key is
+// above lbits); untouched for positions without rows.
+__global__ void k_block_first(int ngroups, const int* __restrict__ off, const unsigned* __restrict__ keys, int lbits,
+                              int* __restrict__ bl) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const unsigned p = keys[off[g]] >> lbits;
+    if (g == 0 || p != (keys[off[g - 1]] >> lbits)) bl[p] = g;
+}
+
 // block_of[row]: block of every row (nblk blocks); pos[b]: processing
 // position of block b.
 void build_block_schedule(Ext* e, bool upper, const int* block_of, int nblk, const std::vector<int>& pos,
@@ -263,25 +274,36 @@ void build_block_schedule(Ext* e, bool upper, const int* block_of, int nblk, con
     e->sort_tmp.reserve(bytes);
     PILU_HIP(rocprim::radix_sort_pairs(e->sort_tmp.get(), bytes, keys.get(), keys_out.get(), rows.get(), perm.get(),
                                        static_cast<unsigned>(n), 0u, end_bit, s));
-    std::vector<unsigned> hk(static_cast<size_t>(n));
-    PILU_HIP(hipMemcpyAsync(hk.data(), keys_out.get(), sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
+    // group starts (a new (block, level) key) selected on the device into
+    // loff, then the first group of every block
+    DevBuf<unsigned char> head(n);
+    DevBuf<unsigned> nsel(1);
+    loff.reserve(static_cast<size_t>(n) + 1);
+    k_group_heads<<<grid_for(n, 256), 256, 0, s>>>(n, keys_out.get(), head.get());
+    PILU_HIP(hipGetLastError());
+    rocprim::counting_iterator<int> idx(0);
+    bytes = 0;
+    PILU_HIP(rocprim::select(nullptr, bytes, idx, head.get(), loff.get(), nsel.get(), static_cast<size_t>(n),
+                             s));
+    e->sort_tmp.reserve(bytes);
+    PILU_HIP(rocprim::select(e->sort_tmp.get(), bytes, idx, head.get(), loff.get(), nsel.get(),
+                             static_cast<size_t>(n), s));
+    unsigned ng = 0;
+    PILU_HIP(hipMemcpyAsync(&ng, nsel.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
     PILU_HIP(hipStreamSynchronize(s));
-    std::vector<int> off, bl(static_cast<size_t>(nblk) + 1, -1);
-    for (int i = 0; i < n; ++i) {
-        if (i == 0 || hk[i] != hk[i - 1]) {
-            const unsigned p = hk[i] >> lbits;
-            if (i == 0 || p != (hk[i - 1] >> lbits)) bl[p] = static_cast<int>(off.size());
-            off.push_back(i);
-        }
-    }
-    off.push_back(n);
-    bl[nblk] = static_cast<int>(off.size()) - 1;
+    const int ngroups = static_cast<int>(ng);
+    k_fill_int<<<1, 64, 0, s>>>(1, loff.get() + ngroups, n);
+    blev.reserve(static_cast<size_t>(nblk) + 1);
+    k_fill_int<<<grid_for(nblk + 1, 256), 256, 0, s>>>(nblk + 1, blev.get(), -1);
+    k_block_first<<<grid_for(ngroups, 256), 256, 0, s>>>(ngroups, loff.get(), keys_out.get(), lbits, blev.get());
+    PILU_HIP(hipGetLastError());
+    std::vector<int> bl(static_cast<size_t>(nblk) + 1);
+    PILU_HIP(hipMemcpyAsync(bl.data(), blev.get(), sizeof(int) * bl.size(), hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    bl[nblk] = ngroups;
     for (int p = nblk - 1; p >= 0; --p)
         if (bl[p] < 0) bl[p] = bl[p + 1];  // a block without rows
-    blev.reserve(bl.size());
-    loff.reserve(off.size());
     PILU_HIP(hipMemcpy(blev.get(), bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice));
-    PILU_HIP(hipMemcpy(loff.get(), off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
 }
 
 // Processing order of the blocks of a partition in one direction: a row
@@ -406,21 +428,38 @@ std::vector<int> block_order(Ext* e, bool upper) {
     return pos;
 }
 
-// Installs the partition in e->h_block_of (blocks 0..nblk-1): uploads it,
-// orders the blocks in both directions and builds both block schedules.
+// Rows of every block.
+__global__ void k_block_counts(int n, const int* __restrict__ block_of, int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(cnt + block_of[i], 1);
+}
+
+__global__ void k_block_contiguous(int n, int blk, int* __restrict__ block_of) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) block_of[i] = i / blk;
+}
+
+// Installs the partition in e->block_of (device, blocks 0..nblk-1): orders
+// the blocks in both directions and builds both block schedules.
 // Contiguous row blocks keep ascending (forward) and descending (backward)
 // order, which k_sweep_blk relies on.
 void install_partition(Ext* e, int nblk, bool contiguous) {
     const int n = e->n;
-    PhaseTimer pt("partition", e->stream);
+    hipStream_t s = e->stream;
+    PhaseTimer pt("partition", s);
     e->nblk = nblk;
-    std::vector<int> cnt(nblk, 0);
-    for (int i = 0; i < n; ++i) ++cnt[e->h_block_of[i]];
-    e->max_block_rows = *std::max_element(cnt.begin(), cnt.end());
+    {
+        DevBuf<int> cnt(std::max(nblk, 1));
+        PILU_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(int) * nblk, s));
+        k_block_counts<<<grid_for(n, 256), 256, 0, s>>>(n, e->block_of.get(), cnt.get());
+        PILU_HIP(hipGetLastError());
+        std::vector<int> hc(nblk);
+        PILU_HIP(hipMemcpyAsync(hc.data(), cnt.get(), sizeof(int) * nblk, hipMemcpyDeviceToHost, s));
+        PILU_HIP(hipStreamSynchronize(s));
+        e->max_block_rows = *std::max_element(hc.begin(), hc.end());
+    }
     if (e->max_block_rows > kMaxBlockRows)
         throw std::runtime_error("sweep partition: a block holds more than 16384 rows");
-    e->block_of.reserve(n);
-    PILU_HIP(hipMemcpy(e->block_of.get(), e->h_block_of.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     std::vector<int> pos_l(nblk), pos_u(nblk);
     if (contiguous) {
         for (int b = 0; b < nblk; ++b) {
@@ -428,7 +467,7 @@ void install_partition(Ext* e, int nblk, bool contiguous) {
             pos_u[b] = nblk - 1 - b;
         }
     } else {
-        pt.mark("upload");
+        pt.mark("block sizes");
         pos_l = block_order(e, false);
         pt.mark("L block order");
         pos_u = block_order(e, true);
@@ -445,8 +484,9 @@ void install_partition(Ext* e, int nblk, bool contiguous) {
 void use_blocks(Ext* e, int blk) {
     if (blk < 64 || blk > kMaxBlockRows) throw std::runtime_error("block rows must be in [64, 16384]");
     if (blk == e->blk_rows && e->tile[0] == 0) return;
-    e->h_block_of.resize(e->n);
-    for (int i = 0; i < e->n; ++i) e->h_block_of[i] = i / blk;
+    e->block_of.reserve(e->n);
+    k_block_contiguous<<<grid_for(e->n, 256), 256, 0, e->stream>>>(e->n, blk, e->block_of.get());
+    PILU_HIP(hipGetLastError());
     e->blk_rows = blk;
     e->tile[0] = e->tile[1] = e->tile[2] = 0;
     install_partition(e, ceil_div(e->n, blk), true);
@@ -495,6 +535,25 @@ __global__ void k_skew(int n, long nx, long ny, const int* __restrict__ rp, cons
 }
 
 // Tiles of the grid declared by SetGridShape (row x + nx (y + ny z)).
+// Tile of every row (row r at x + nx (y + ny z)) and, by atomicMin, the
+// first row of every tile.
+__global__ void k_tile_of(int n, long nx, long ny, long b, long c, long c2, int tx, int ty, int tz, long ntu,
+                          long ntv, int* __restrict__ tile_of, int* __restrict__ first) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const long x = r % nx, yz = r / nx, y = yz % ny, z = yz / ny;
+    const long u = x + b * y + c * z, v = y + c2 * z;
+    const int t = static_cast<int>(((z / tz) * ntv + v / ty) * ntu + u / tx);
+    tile_of[r] = t;
+    atomicMin(first + t, r);
+}
+
+__global__ void k_apply_tile_id(int n, const int* __restrict__ tile_of, const int* __restrict__ id,
+                                int* __restrict__ block_of) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) block_of[r] = id[tile_of[r]];
+}
+
 // Tiles are boxes of tx x ty x tz in skewed coordinates
 //   u = x + b y + c z,  v = y + c2 z,  w = z,
 // with the smallest non-negative integers b, c, c2 for which no forward
@@ -533,35 +592,30 @@ void use_tiles(Ext* e, int tx, int ty, int tz) {
         }
     }
     pt.mark("skew");
-    const GridCoord gc(nx, ny);
     const long umax = (nx - 1) + b * (ny - 1) + c * (nz - 1), vmax = (ny - 1) + c2 * (nz - 1);
     const long ntu = umax / tx + 1, ntv = vmax / ty + 1, ntw = (nz - 1) / tz + 1;
     if (ntu * ntv * ntw > (1L << 30)) throw std::runtime_error("tiled sweeps: too many tiles");
-    // dense ids of the non-empty tiles, in order of their first row
-    std::vector<long> tile_of(static_cast<size_t>(n));
-    parallel_for(n, [&](long lo, long hi) {
-        long x, y, z;
-        gc(lo, x, y, z);
-        for (long r = lo; r < hi; ++r) {
-            const long u = x + b * y + c * z, v = y + c2 * z;
-            tile_of[r] = ((z / tz) * ntv + v / ty) * ntu + u / tx;
-            if (++x == nx) {
-                x = 0;
-                if (++y == ny) {
-                    y = 0;
-                    ++z;
-                }
-            }
-        }
-    });
-    std::vector<int> id(static_cast<size_t>(ntu * ntv * ntw), -1);
-    e->h_block_of.resize(n);
-    int nblk = 0;
-    for (int r = 0; r < n; ++r) {
-        const long t = tile_of[r];
-        if (id[t] < 0) id[t] = nblk++;
-        e->h_block_of[r] = id[t];
-    }
+    const int ntiles = static_cast<int>(ntu * ntv * ntw);
+    // tile of every row and the first row of every tile on the device; the
+    // non-empty tiles get dense ids in order of their first row
+    DevBuf<int> tile_of(std::max(n, 1)), first(ntiles), id(ntiles);
+    k_fill_int<<<grid_for(ntiles, 256 * 8), 256, 0, st>>>(ntiles, first.get(), INT_MAX);
+    k_tile_of<<<grid_for(n, 256), 256, 0, st>>>(n, nx, ny, b, c, c2, tx, ty, tz, ntu, ntv, tile_of.get(), first.get());
+    PILU_HIP(hipGetLastError());
+    std::vector<int> hf(ntiles), hid(ntiles, -1);
+    PILU_HIP(hipMemcpyAsync(hf.data(), first.get(), sizeof(int) * ntiles, hipMemcpyDeviceToHost, st));
+    PILU_HIP(hipStreamSynchronize(st));
+    std::vector<std::pair<int, int>> order;
+    for (int t = 0; t < ntiles; ++t)
+        if (hf[t] != INT_MAX) order.emplace_back(hf[t], t);
+    std::sort(order.begin(), order.end());
+    const int nblk = static_cast<int>(order.size());
+    for (int k = 0; k < nblk; ++k) hid[order[k].second] = k;
+    PILU_HIP(hipMemcpyAsync(id.get(), hid.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, st));
+    e->block_of.reserve(n);
+    k_apply_tile_id<<<grid_for(n, 256), 256, 0, st>>>(n, tile_of.get(), id.get(), e->block_of.get());
+    PILU_HIP(hipGetLastError());
+    PILU_HIP(hipStreamSynchronize(st));  // hid goes out of scope
     e->blk_rows = 0;
     e->tile[0] = tx;
     e->tile[1] = ty;
@@ -573,8 +627,30 @@ void use_tiles(Ext* e, int tx, int ty, int tz) {
     install_partition(e, nblk, false);
 }
 
+// Longest row, L part and U part: ext[0..2] (atomic maxima).
+__global__ void k_row_extents(int n, const int* __restrict__ rp, const int* __restrict__ dg, int* ext) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = 0, lo = 0, up = 0;
+    if (i < n) {
+        r = rp[i + 1] - rp[i];
+        lo = dg[i] - rp[i];
+        up = rp[i + 1] - dg[i] - 1;
+    }
+    // wave maxima first: one atomic per wave and extent
+    for (int o = 32; o > 0; o >>= 1) {
+        r = max(r, __shfl_xor(r, o, 64));
+        lo = max(lo, __shfl_xor(lo, o, 64));
+        up = max(up, __shfl_xor(up, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(ext, r);
+        atomicMax(ext + 1, lo);
+        atomicMax(ext + 2, up);
+    }
+}
+
 // Device pattern is in e->rp / e->ci: validate, locate diagonals, schedule.
-void analyse(Ext* e, const int* host_rp) {
+void analyse(Ext* e) {
     const int n = e->n;
     PhaseTimer pt("analyse", e->stream);
     e->analysed = false;  // until every schedule below is built
@@ -600,33 +676,24 @@ void analyse(Ext* e, const int* host_rp) {
     if (flags[0]) throw std::runtime_error("rows must have strictly increasing, non-negative column indices");
     if (flags[1]) throw std::runtime_error("missing diag");
 
-    int max_row = 0, max_lo = 0, max_up = 0;
-    {
-        std::vector<int>& h = e->h_rp;
-        std::vector<int>& hd = e->h_dg;
-        h.resize(static_cast<size_t>(n) + 1);
-        hd.resize(static_cast<size_t>(n));
-        e->h_ci.clear();  // columns are copied again on first use
-        if (host_rp) std::copy(host_rp, host_rp + n + 1, h.begin());
-        else PILU_HIP(hipMemcpy(h.data(), e->rp.get(), h.size() * sizeof(int), hipMemcpyDeviceToHost));
-        PILU_HIP(hipMemcpy(hd.data(), e->dg.get(), sizeof(int) * static_cast<size_t>(n), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) {
-            max_row = std::max(max_row, h[i + 1] - h[i]);
-            max_lo = std::max(max_lo, hd[i] - h[i]);
-            max_up = std::max(max_up, h[i + 1] - hd[i] - 1);
-        }
-    }
-    e->max_row = max_row;
-    e->max_lo = max_lo;
-    e->max_up = max_up;
-    pt.mark("pattern checks, diagonal, host row pointers");
+    // longest row, longest L part and longest U part (device reduction)
+    PILU_HIP(hipMemsetAsync(e->err.get() + kErrExtent, 0, 3 * sizeof(int), s));
+    k_row_extents<<<grid_for(n, 256), 256, 0, s>>>(n, e->rp.get(), e->dg.get(), e->err.get() + kErrExtent);
+    PILU_HIP(hipGetLastError());
+    int ext[3] = {0, 0, 0};
+    PILU_HIP(hipMemcpyAsync(ext, e->err.get() + kErrExtent, sizeof(ext), hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    e->max_row = ext[0];
+    e->max_lo = ext[1];
+    e->max_up = ext[2];
+    ++e->pattern_version;  // host copies of the pattern are stale from here on
+    pt.mark("pattern checks, diagonal, row extents");
 
     (void)device_scratch();  // the no-allocation apply's scratch on this device
     e->nlev_l = build_schedule(e, false, e->perm_l, e->lvoff_l);
     pt.mark("L level schedule");
     e->nlev_u = build_schedule(e, true, e->perm_u, e->lvoff_u);
     pt.mark("U level schedule");
-    ++e->pattern_version;
 
     e->vin.reserve(e->nnz);
     e->fac.reserve(e->nnz);
@@ -1783,7 +1850,7 @@ void IluSolver::SetupMatrix() {
                        hipMemcpyHostToDevice));
     if (nnz > 0)
         PILU_HIP(hipMemcpy(e->ci.get(), aMatrix_.col_idx, sizeof(int) * nnz, hipMemcpyHostToDevice));
-    pilu::analyse(e, aMatrix_.row_ptr);
+    pilu::analyse(e);
     pilu::show_algorithm(this);
 }
 
@@ -1800,7 +1867,7 @@ void IluSolver::SetupMatrixDevice(int n, int nnz, const int* d_row_ptr, const in
                             hipMemcpyDeviceToDevice, e->stream));
     PILU_HIP(hipMemcpyAsync(e->ci.get(), d_col_idx, sizeof(int) * static_cast<size_t>(nnz),
                             hipMemcpyDeviceToDevice, e->stream));
-    pilu::analyse(e, nullptr);
+    pilu::analyse(e);
 }
 
 bool IluSolver::Factorize() {
@@ -1845,7 +1912,7 @@ int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx,
     e->ci = std::move(res.ci);
     e->nzmap = std::move(res.nzmap);
     e->fill_level = std::move(res.lev);
-    pilu::analyse(e, nullptr);
+    pilu::analyse(e);
     pilu::show_algorithm(this);
     return nnz_f;
 }
