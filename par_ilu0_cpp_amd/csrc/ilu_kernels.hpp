@@ -107,23 +107,51 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
     int lv = 0;
     int top = -1;  // highest level published by this lane
     bool done = !active;
-    // One lane per row: the row's first kLevE dependencies (columns and
-    // block test) are read once and their levels polled together, so a
-    // chain of dependent rows costs one round trip per row instead of two
-    // per dependency; further entries go through the loop below.
+    // One lane per row (G == 1): the row's first kLevE dependencies (columns
+    // and block test) are read once.  Those solved by a lane of the same
+    // wave (the previous rows in the order) are taken from that lane's
+    // register by a cross-lane read, repeated while some lane of the wave
+    // finishes, so a chain of rows inside a wave costs no memory round trip;
+    // the others are polled together in memory, one round trip per row
+    // instead of two per dependency.  Further entries go through the loop
+    // below (memory).
     constexpr int kLevE = G == 1 ? 8 : 1;
-    int hc[kLevE];
+    int hc[kLevE];  // dependency polled in memory, or -1
+    int hl[kLevE];  // lane of this wave solving the dependency, or -1 (compacted to the front)
     bool head = G == 1;
+    int pub = -1;   // this lane's level once known (read by later lanes of the wave)
+    int nl_max = 0; // most in-wave dependencies of a lane of the wave
     if constexpr (G == 1) {
+        const long k0 = k - lane;  // schedule position of the wave's first row
+        int nl = 0;
 #pragma unroll
         for (int q = 0; q < kLevE; ++q) {
             hc[q] = -1;
+            hl[q] = -1;
+        }
+#pragma unroll
+        for (int q = 0; q < kLevE; ++q) {
             if (active && e + q < end) {
                 const int c = ci[e + q];
-                hc[q] = block_of && block_of[c] != my_block ? -1 : c;
+                if (!(block_of && block_of[c] != my_block)) {
+                    const long kc = UPPER ? static_cast<long>(n) - 1 - c : static_cast<long>(c);
+                    if (kc >= k0 && kc < k0 + 64) {
+#pragma unroll
+                        for (int x = 0; x < kLevE; ++x)
+                            if (x == nl) hl[x] = static_cast<int>(kc - k0);
+                        ++nl;
+                    } else {
+                        hc[q] = c;
+                    }
+                }
             }
         }
         e = e + kLevE < end ? e + kLevE : end;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_xor(nl, off, 64);
+            nl = o > nl ? o : nl;
+        }
+        nl_max = nl;
     }
     while (true) {
         if (!done) {
@@ -157,8 +185,39 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
                 atomicOr(err, kErrTimeout);
                 e = end;
                 head = false;
+#pragma unroll
+                for (int q = 0; q < kLevE; ++q) hl[q] = -1;
             }
-            if (group_all<G>(!head && e >= end)) {
+        }
+        if constexpr (G == 1) {
+            // in-wave dependencies from the solving lanes' registers, while
+            // some lane of the wave finishes (at most 64 passes)
+            for (int it = 0; it < 64; ++it) {
+                bool fin = false;
+#pragma unroll
+                for (int q = 0; q < kLevE; ++q) {
+                    if (q < nl_max) {  // uniform over the wave
+                        const int d = __shfl(pub, hl[q] >= 0 ? hl[q] : lane, 64);
+                        if (hl[q] >= 0 && d >= 0) {
+                            lv = d + 1 > lv ? d + 1 : lv;
+                            hl[q] = -1;
+                        }
+                    }
+                }
+                bool inw = true;
+#pragma unroll
+                for (int q = 0; q < kLevE; ++q) inw = inw && hl[q] < 0;
+                if (!done && !head && e >= end && inw) {
+                    pub = lv;
+                    publish_int(level + row, lv);
+                    top = lv > top ? lv : top;
+                    done = true;
+                    fin = true;
+                }
+                if (nl_max == 0 || __ballot(fin) == 0) break;
+            }
+        } else {
+            if (!done && group_all<G>(!head && e >= end)) {
                 const int l = group_max<G>(lv);
                 if (t == 0) {
                     publish_int(level + row, l);
