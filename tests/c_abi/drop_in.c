@@ -5,22 +5,32 @@
  * b = A * ones, set up, factorize, GMRES with the default parameters from
  * x0 = 0, error against the exact solution.
  *
- * Output lines: "setup <rc>", "factorize <rc>", "iterations <n>",
+ * With a second argument "mkl" it also takes the path of the reference's
+ * USE_MKL build through the Ex ABI: dcsrilu0's pivot check (ipar[30] = 0)
+ * before the factorization, and y = U^-1 (L^-1 b) as two single triangular
+ * solves after it.
+ *
+ * Output lines: "setup <rc>", "factorize <rc>", ["factor_status <code>",
+ * "trsv <y[0]> <y[n/2]> <y[n-1]> <max |y|>",] "iterations <n>",
  * "converged <0|1>", "max_error <e>", "timers <init> <fgmr> <mv> <precon>".
  * Exit status: 0 converged; 2 GMRES failed before max_iterations; 3 not
- * converged; 4 setup or factorization failed; 1 usage. */
+ * converged; 4 setup or factorization failed; 5 a triangular solve failed;
+ * 1 usage. */
 #include <math.h>
 #include <stdbool.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "gmres_c.h"
+#include "par_ilu_ex.h"
 
 int main(int argc, char** argv) {
-    if (argc != 2) {
-        fprintf(stderr, "usage: %s matrix.mtx\n", argv[0]);
+    if (argc < 2 || argc > 3 || (argc == 3 && strcmp(argv[2], "mkl") != 0)) {
+        fprintf(stderr, "usage: %s matrix.mtx [mkl]\n", argv[0]);
         return 1;
     }
+    const bool mkl = argc == 3;
     CsrMatrix a = CSR_MATRIX_DEFAULT;
     ReadCsrMatrixMM1(&a, argv[1]);
     const int n = a.size;
@@ -38,11 +48,28 @@ int main(int argc, char** argv) {
     GmresHdl gmres = NULL;
     const int rs = IluSolverSetup(ilu);
     printf("setup %d\n", rs);
+    if (mkl && rs == 0) IluSolverSetPivotPolicyEx(ilu, 1, 1e-16, 1e-10);
     const int rf = rs == 0 ? IluSolverFactorize(ilu) : -1;
     printf("factorize %d\n", rf);
+    if (mkl) printf("factor_status %d\n", IluSolverFactorStatusEx(ilu));
     if (rs != 0 || rf != 0) {
         status = 4;
         goto done;
+    }
+    if (mkl) {
+        double* t = (double*)malloc(sizeof(double) * (size_t)n);
+        double* y = (double*)malloc(sizeof(double) * (size_t)n);
+        const int r1 = IluSolverTrsvEx(ilu, 0, 1.0, b, t);
+        const int r2 = IluSolverTrsvEx(ilu, 1, 1.0, t, y);
+        double big = 0.0;
+        for (int i = 0; i < n; ++i) big = fmax(big, fabs(y[i]));
+        printf("trsv %.17e %.17e %.17e %.17e\n", y[0], y[n / 2], y[n - 1], big);
+        free(t);
+        free(y);
+        if (r1 != 0 || r2 != 0) {
+            status = 5;
+            goto done;
+        }
     }
 
     gmres = GmresCreate();

@@ -75,6 +75,37 @@ def test_c_program_solves_like_the_reference_driver(oracle, tmp_path, case):
     assert all(float(t) >= 0 for t in f["timers"]) and float(f["timers"][3]) > 0
 
 
+@pytest.mark.gpu
+def test_c_program_mkl_mode(oracle, tmp_path):
+    """The USE_MKL flow from C: dcsrilu0-style pivot check, two single
+    triangular solves composing to the preconditioner, then the solve."""
+    a = gen.convdiff3d(12)
+    mtx = tmp_path / "a.mtx"
+    write_mm(mtx, a)
+    exe = build(tmp_path)
+    r = subprocess.run([str(exe), str(mtx), "mkl"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    f = fields(r.stdout)
+    assert f["factorize"] == ["0"] and f["factor_status"] == ["0"] and f["converged"] == ["1"]
+    b = a.matvec(np.ones(a.n))
+    fac, dg, _ = oracle.ilu_factorize(a.row_ptr, a.col_idx, a.value)
+    y = oracle.substitute(a.row_ptr, a.col_idx, dg, fac, b)
+    got = [float(v) for v in f["trsv"]]
+    want = [y[0], y[a.n // 2], y[-1], np.max(np.abs(y))]
+    assert np.allclose(got, want, rtol=1e-12, atol=0), (got, want)
+    # [[1, 1], [1, 1]] in the leading block (nonzero diagonal, so the
+    # reader keeps the rows): u_22 = 1 - 1 * 1 = 0 stops the factorization
+    # with dcsrilu0's -102
+    rp = np.array([0, 2, 4] + list(range(5, 13)), dtype=np.int32)
+    ci = np.array([0, 1, 0, 1] + list(range(2, 10)), dtype=np.int32)
+    va = np.array([1.0, 1.0, 1.0, 1.0] + [2.0] * 8)
+    z = tmp_path / "z.mtx"
+    write_mm(z, gen.Csr(10, rp, ci, va))
+    r = subprocess.run([str(exe), str(z), "mkl"], capture_output=True, text=True, timeout=300)
+    f = fields(r.stdout)
+    assert r.returncode == 4 and f["factorize"] == ["-1"] and f["factor_status"] == ["-102"], r.stdout
+
+
 CPP = ROOT / "tests" / "c_abi" / "reference_layout.cpp"
 
 
