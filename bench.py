@@ -75,10 +75,11 @@ def parse():
                         "value-flags for FP32 factors (mode 3)")
     p.add_argument("--traffic", default=str(ROOT / "profiles" / "sweep_traffic_convdiff128.txt"),
                    help="rocprofv3 summary holding the measured HBM bytes per apply")
-    p.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"),
-                   help="c2: the benchmark line (128^3 convection-diffusion); c3: random n=1e6 ILU(2)-GMRES(40); "
-                        "c4: 512 batched 2D diffusion systems x 8 right-hand sides, ILU(0)-GMRES(30).  c3 and c4 "
-                        "print one JSON line of their own")
+    p.add_argument("--workload", default="c2", choices=("c2", "c2-host", "c3", "c4"),
+                   help="c2: the benchmark line (128^3 convection-diffusion); c2-host: the same solve through the "
+                        "host-buffer API (PCIe-inclusive); c3: random n=1e6 ILU(2)-GMRES(40); c4: 512 batched 2D "
+                        "diffusion systems x 8 right-hand sides, ILU(0)-GMRES(30).  c2-host, c3 and c4 print one "
+                        "JSON line of their own")
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a 1 GiB device copy three times (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
@@ -154,6 +155,9 @@ def main():
         return
     if args.workload == "c4":
         workload_c4()
+        return
+    if args.workload == "c2-host":
+        workload_c2_host(args.side)
         return
 
     import numpy as np
@@ -444,6 +448,64 @@ def workload_c3(n=1_000_000, lof=2, m=40, sweep=""):
         del g, ilu
     out["fp32_vs_fp64_rel_err"] = float(np.linalg.norm(sols[32] - sols[64]) / np.linalg.norm(sols[64]))
     print(json.dumps(out), flush=True)
+
+
+def workload_c2_host(s=128, reps=3):
+    """The benchmark's solve through the host-buffer API, as the reference's
+    own driver calls it (test/benchmark.cpp:98-131): IlukSymbolic on host
+    arrays, SetupMatrix, then per value set the nzmap refresh on the host,
+    Factorize (values up, factors back into F.value) and GmresSolve (A and b
+    up, x down).  FP64 factors, tiles 128x4x8 as in the device-resident
+    line; every transfer is inside the timed region."""
+    import numpy as np
+
+    import par_ilu0_cpp_amd as P
+    from par_ilu0_cpp_amd import generators as gen
+
+    a = gen.convdiff3d(s)
+    b = gen.rhs_uniform(a.n, seed=0)
+    h = P.HostCsr.from_csr(a)
+    f = P.HostCsr()
+    nz = np.zeros(a.nnz, dtype=np.int32)
+    t0 = time.perf_counter()
+    nnz_f = P.IlukSymbolic(h, 1, f.mat, nz)
+    t_sym = time.perf_counter() - t0
+    frp, fci, fva = f.arrays()
+    ilu = P.IluSolver(0)
+    ilu.SetMatrix(frp, fci, fva)
+    t0 = time.perf_counter()
+    if ilu.SetupMatrix() != 0:
+        raise SystemExit("SetupMatrix failed")
+    ilu.SetGridShape(s, s, s)
+    ilu.SetSweepLanes(8, 8)
+    ilu.SetSweepTiles(2, 2, min(s, 128), 4, 8)
+    t_setup = time.perf_counter() - t0
+    g = P.PreconditionedGmres()
+    g.SetIluPreconditioner(ilu)
+    prm = g.param()
+    prm.tolerance = 1e-8 * float(np.linalg.norm(b))
+    prm.krylov_subspace_dimension = 50
+    prm.max_iterations = 2000
+    fv = P._view(ilu.GetMatrix().value, nnz_f, np.float64)
+    runs = []
+    for rep in range(reps):
+        fv[:] = 0.0
+        fv[nz] = a.value  # test/benchmark.cpp:116-119
+        x = np.zeros(a.n)
+        t0 = time.perf_counter()
+        rc = ilu.Factorize()
+        t1 = time.perf_counter()
+        ok, its = g.Solve(h, b, x)
+        t2 = time.perf_counter()
+        res = float(np.linalg.norm(b - a.matvec(x)) / np.linalg.norm(b))
+        runs.append({"factorize_s": t1 - t0, "solve_s": t2 - t1, "total_s": t2 - t0, "its": its,
+                     "ok": bool(ok and rc == 0), "true_rel_res": res})
+        log(f"[c2-host] rep {rep}: {runs[-1]}")
+    best = min(runs[1:] or runs, key=lambda r: r["total_s"])
+    print(json.dumps({"workload": f"{s}^3 conv-diff ILU(1)-GMRES(50), host buffers (PCIe-inclusive)",
+                      "symbolic_host_s": t_sym, "setup_s": t_setup, "per_value_set_s": best["total_s"],
+                      "factorize_s": best["factorize_s"], "gmres_s": best["solve_s"], "its": best["its"],
+                      "true_rel_res": best["true_rel_res"], "runs": runs}), flush=True)
 
 
 def workload_c4(count=512, nrhs=8, m=30):
