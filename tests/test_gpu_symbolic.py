@@ -48,7 +48,8 @@ def test_matches_oracle_random(oracle, lof, shape):
 
 
 @pytest.mark.parametrize("lof", [0, 1])
-@pytest.mark.parametrize("case", ["random", "convdiff", "poisson", "unsorted", "nodiag", "small"])
+@pytest.mark.parametrize("case", ["random", "convdiff", "poisson", "unsorted", "nodiag", "small", "random_big",
+                                  "convdiff_big"])
 def test_rows_from_a_match_oracle(oracle, case, lof):
     """With lof <= 1 and sorted rows of A the pass reads pivot rows from A
     (no row waits for another); with a row of A out of column order or
@@ -71,6 +72,11 @@ def test_rows_from_a_match_oracle(oracle, case, lof):
         rp = np.concatenate([[0], np.cumsum([keep[rp[i]:rp[i + 1]].sum() for i in range(5000)])]).astype(np.int32)
     elif case == "small":
         rp, ci = random_pattern(300, 12, 299, seed=14)
+    elif case == "random_big":  # more rows than waves: each wave strides over several rows
+        rp, ci = random_pattern(40000, 6, 500, seed=15)
+    elif case == "convdiff_big":
+        a = gen.convdiff3d(30)
+        rp, ci = a.row_ptr, a.col_idx
     else:
         rp, ci = random_pattern(5000, 5, 200, seed=12)
         ci = ci.copy()
