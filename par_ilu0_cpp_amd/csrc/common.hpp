@@ -22,7 +22,23 @@ namespace pilu {
         if (pilu_e_ != hipSuccess) ::pilu::hip_fail(pilu_e_, #call, __FILE__, __LINE__); \
     } while (0)
 
-// Owning device allocation (hipMalloc / hipFree).
+// Device allocation behind DevBuf.  Inside a SetupScope (per host thread)
+// freed blocks are kept and handed out again (best fit, at most twice the
+// request) instead of going back to the driver; the outermost scope's end
+// synchronises its stream and frees them.  All work on such blocks inside a
+// scope must be ordered on that one stream.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p, size_t bytes);
+
+class SetupScope {
+public:
+    explicit SetupScope(hipStream_t s);
+    ~SetupScope();
+    SetupScope(const SetupScope&) = delete;
+    SetupScope& operator=(const SetupScope&) = delete;
+};
+
+// Owning device allocation (dev_alloc / dev_free).
 template <typename T>
 class DevBuf {
 public:
@@ -40,11 +56,11 @@ public:
     void reserve(size_t count) { if (count > n_) resize(count); }
     void resize(size_t count) {
         release();
-        if (count) PILU_HIP(hipMalloc(reinterpret_cast<void**>(&p_), count * sizeof(T)));
+        if (count) p_ = static_cast<T*>(dev_alloc(count * sizeof(T)));
         n_ = count;
     }
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_) dev_free(p_, n_ * sizeof(T));
         p_ = nullptr;
         n_ = 0;
     }

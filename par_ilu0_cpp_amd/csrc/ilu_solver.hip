@@ -36,6 +36,58 @@ namespace pilu {
     throw std::runtime_error(buf);
 }
 
+namespace {
+// Blocks freed inside the current thread's SetupScope.
+struct ScopeCache {
+    int depth = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::pair<void*, size_t>> blocks;
+};
+thread_local ScopeCache g_scope;
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+    if (g_scope.depth > 0) {
+        size_t best = 0;
+        int at = -1;
+        for (size_t i = 0; i < g_scope.blocks.size(); ++i) {
+            const size_t b = g_scope.blocks[i].second;
+            if (b >= bytes && b / 2 <= bytes && (at < 0 || b < best)) {
+                best = b;
+                at = static_cast<int>(i);
+            }
+        }
+        if (at >= 0) {
+            void* p = g_scope.blocks[at].first;
+            g_scope.blocks.erase(g_scope.blocks.begin() + at);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    PILU_HIP(hipMalloc(&p, bytes));
+    return p;
+}
+
+void dev_free(void* p, size_t bytes) {
+    if (!p) return;
+    if (g_scope.depth > 0) {
+        g_scope.blocks.emplace_back(p, bytes);
+        return;
+    }
+    (void)hipFree(p);
+}
+
+SetupScope::SetupScope(hipStream_t s) {
+    if (g_scope.depth++ == 0) g_scope.stream = s;
+}
+
+SetupScope::~SetupScope() {
+    if (--g_scope.depth > 0 || g_scope.blocks.empty()) return;
+    (void)hipStreamSynchronize(g_scope.stream);  // no kernel may still use a block
+    for (auto& b : g_scope.blocks) (void)hipFree(b.first);
+    g_scope.blocks.clear();
+}
+
 int lanes_for_avg(double avg_len) {
     int g = 4;
     while (g < 64 && g < avg_len) g <<= 1;
@@ -428,10 +480,18 @@ std::vector<int> block_order(Ext* e, bool upper) {
     return pos;
 }
 
-// Rows of every block.
+// Rows of every block.  Consecutive rows mostly share a block: a wave whose
+// rows all lie in one block adds them with one atomic.
 __global__ void k_block_counts(int n, const int* __restrict__ block_of, int* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(cnt + block_of[i], 1);
+    const int b = i < n ? block_of[i] : -1;
+    const int b0 = __shfl(b, 0, 64);
+    const unsigned long long act = __ballot(i < n);
+    if (__ballot(i < n && b != b0) == 0) {
+        if ((threadIdx.x & 63) == 0 && b0 >= 0) atomicAdd(cnt + b0, __popcll(act));
+    } else if (i < n) {
+        atomicAdd(cnt + b, 1);
+    }
 }
 
 __global__ void k_block_contiguous(int n, int blk, int* __restrict__ block_of) {
@@ -545,7 +605,14 @@ __global__ void k_tile_of(int n, long nx, long ny, long b, long c, long c2, int 
     const long u = x + b * y + c * z, v = y + c2 * z;
     const int t = static_cast<int>(((z / tz) * ntv + v / ty) * ntu + u / tx);
     tile_of[r] = t;
-    atomicMin(first + t, r);
+    // a wave whose rows all lie in one tile: one atomic with its first row
+    // (lane 0's, the smallest)
+    const int t0 = __shfl(t, 0, 64);
+    if (__ballot(t != t0) == 0) {
+        if ((threadIdx.x & 63) == 0) atomicMin(first + t0, r);
+    } else {
+        atomicMin(first + t, r);
+    }
 }
 
 __global__ void k_apply_tile_id(int n, const int* __restrict__ tile_of, const int* __restrict__ id,
@@ -1834,6 +1901,7 @@ IluSolver::~IluSolver() {
 }
 
 void IluSolver::SetupMatrix() {
+    pilu::SetupScope scope(nullptr);
     const int n = aMatrix_.size;
     if (n <= 0 || !aMatrix_.row_ptr) throw std::runtime_error("IluSolver: empty matrix");
     const long nnz = aMatrix_.row_ptr[n];
@@ -1855,6 +1923,7 @@ void IluSolver::SetupMatrix() {
 }
 
 void IluSolver::SetupMatrixDevice(int n, int nnz, const int* d_row_ptr, const int* d_col_idx) {
+    pilu::SetupScope scope(nullptr);
     if (n <= 0) throw std::runtime_error("IluSolver: empty matrix");
     if (!ext_) ext_ = new Ext;
     Ext* e = ext_;
@@ -1894,6 +1963,7 @@ bool IluSolver::FactorizeDevice(const double* d_values) {
 }
 
 int IluSolver::SymbolicDevice(int n, const int* d_row_ptr, const int* d_col_idx, int lof) {
+    pilu::SetupScope scope(nullptr);
     if (!ext_) ext_ = new Ext;
     Ext* e = ext_;
     e->analysed = false;  // the previous pattern is gone whether or not this completes
@@ -2025,6 +2095,7 @@ namespace lljbash {
 constexpr int kLvTryRows = 256;
 
 int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
+    pilu::SetupScope scope(nullptr);
     Ext* e = pilu::need_setup(ext_);
     if (!e->factored) throw std::runtime_error("IluSolver: Factorize before AutotuneSweeps");
     reps = std::max(reps, 1);
@@ -2156,6 +2227,7 @@ int IluSolver::AutotuneSweeps(int reps, double* table, int cap) {
 }
 
 void IluSolver::SetSweepMode(int mode_l, int mode_u, int blk_rows) {
+    pilu::SetupScope scope(nullptr);
     Ext* e = pilu::need_setup(ext_);
     if (mode_l < 0 || mode_l > 5 || mode_u < 0 || mode_u > 5)
         throw std::runtime_error("sweep mode is 0, 1, 2, 3, 4 or 5");
@@ -2177,6 +2249,7 @@ void IluSolver::SetGridShape(int nx, int ny, int nz) {
 }
 
 void IluSolver::SetSweepTiles(int mode_l, int mode_u, int tx, int ty, int tz) {
+    pilu::SetupScope scope(nullptr);
     Ext* e = pilu::need_setup(ext_);
     auto ok = [](int m) { return m == 0 || m == 2 || m == 3 || m == 5; };
     if (!ok(mode_l) || !ok(mode_u)) throw std::runtime_error("tiled sweeps use mode 0, 2, 3 or 5");

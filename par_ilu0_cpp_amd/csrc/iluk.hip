@@ -359,8 +359,10 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
     for (int attempt = 0;; ++attempt) {
         if (short_rows) cap = std::max<long>(cap, static_cast<long>(n) * kSymCapShort);
         pool.resize(cap);
-        check(hipMemsetAsync(pool.get(), 0xFF, sizeof(uint32_t) * cap, s), "pool fill");
-        check(hipMemsetAsync(header.get(), 0xFF, sizeof(uint64_t) * n, s), "header fill");
+        if (!a_pivots) {  // the waiting form polls the pool and the headers
+            check(hipMemsetAsync(pool.get(), 0xFF, sizeof(uint32_t) * cap, s), "pool fill");
+            check(hipMemsetAsync(header.get(), 0xFF, sizeof(uint64_t) * n, s), "header fill");
+        }
         check(hipMemsetAsync(misc.get(), 0, 8 * sizeof(int), s), "misc");
         check(hipMemsetAsync(top.get(), 0, sizeof(unsigned long long), s), "top");
         const int grid = a_pivots ? std::min(ceil_div(n, kSymWaves), kSymGridA) : ceil_div(n, kSymWaves);
@@ -429,6 +431,7 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
 
 extern "C" int IlukSymbolic(const CsrMatrix* a, int lof, CsrMatrix* f, int* nzmap) {
     try {
+        pilu::SetupScope scope(nullptr);
         const int n = a->size;
         if (n <= 0) return -1;
         const int nnz_a = a->row_ptr[n];
