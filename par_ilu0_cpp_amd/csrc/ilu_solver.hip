@@ -480,18 +480,22 @@ std::vector<int> block_order(Ext* e, bool upper) {
     return pos;
 }
 
-// Rows of every block.  Consecutive rows mostly share a block: a wave whose
-// rows all lie in one block adds them with one atomic.
+// Rows of every block.  Consecutive rows mostly share a block, and a wave
+// of them spans at most two blocks of a tiled or contiguous partition: the
+// lanes in the block of the wave's first row add with one atomic, those in
+// the block of its last row with another, any other lane on its own.
 __global__ void k_block_counts(int n, const int* __restrict__ block_of, int* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = i < n ? block_of[i] : -1;
-    const int b0 = __shfl(b, 0, 64);
-    const unsigned long long act = __ballot(i < n);
-    if (__ballot(i < n && b != b0) == 0) {
-        if ((threadIdx.x & 63) == 0 && b0 >= 0) atomicAdd(cnt + b0, __popcll(act));
-    } else if (i < n) {
-        atomicAdd(cnt + b, 1);
-    }
+    const int lane = threadIdx.x & 63;
+    const bool act = i < n;
+    const int b = act ? block_of[i] : -1;
+    const int last = 63 - __clzll(__ballot(act));  // highest active lane (act lanes are the low ones)
+    const int b0 = __shfl(b, 0, 64), b1 = __shfl(b, last < 0 ? 0 : last, 64);
+    const unsigned long long m0 = __ballot(act && b == b0);
+    const unsigned long long m1 = __ballot(act && b == b1 && b != b0);
+    if (lane == 0 && m0) atomicAdd(cnt + b0, __popcll(m0));
+    if (m1 && lane == __ffsll(static_cast<long long>(m1)) - 1) atomicAdd(cnt + b1, __popcll(m1));
+    if (act && b != b0 && b != b1) atomicAdd(cnt + b, 1);
 }
 
 __global__ void k_block_contiguous(int n, int blk, int* __restrict__ block_of) {
@@ -605,14 +609,16 @@ __global__ void k_tile_of(int n, long nx, long ny, long b, long c, long c2, int 
     const long u = x + b * y + c * z, v = y + c2 * z;
     const int t = static_cast<int>(((z / tz) * ntv + v / ty) * ntu + u / tx);
     tile_of[r] = t;
-    // a wave whose rows all lie in one tile: one atomic with its first row
-    // (lane 0's, the smallest)
-    const int t0 = __shfl(t, 0, 64);
-    if (__ballot(t != t0) == 0) {
-        if ((threadIdx.x & 63) == 0) atomicMin(first + t0, r);
-    } else {
-        atomicMin(first + t, r);
-    }
+    // a wave spans at most two tiles mostly: one atomic for the tile of its
+    // first row (lane 0's row, the smallest), one for the tile of its last
+    // row (its first lane there), any other lane on its own
+    const int lane = threadIdx.x & 63;
+    const int last = 63 - __clzll(__ballot(true));
+    const int t0 = __shfl(t, 0, 64), t1 = __shfl(t, last, 64);
+    const unsigned long long m1 = __ballot(t == t1 && t != t0);
+    if (lane == 0) atomicMin(first + t0, r);
+    if (m1 && lane == __ffsll(static_cast<long long>(m1)) - 1) atomicMin(first + t1, r);
+    if (t != t0 && t != t1) atomicMin(first + t, r);
 }
 
 __global__ void k_apply_tile_id(int n, const int* __restrict__ tile_of, const int* __restrict__ id,
@@ -694,25 +700,33 @@ void use_tiles(Ext* e, int tx, int ty, int tz) {
     install_partition(e, nblk, false);
 }
 
-// Longest row, L part and U part: ext[0..2] (atomic maxima).
-__global__ void k_row_extents(int n, const int* __restrict__ rp, const int* __restrict__ dg, int* ext) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Longest row, L part and U part: ext[0..2] (atomic maxima).  A grid of at
+// most 1024 workgroups strides over the rows; each reduces in registers and
+// LDS and adds one atomic per extent.
+__global__ __launch_bounds__(256) void k_row_extents(int n, const int* __restrict__ rp, const int* __restrict__ dg,
+                                                     int* ext) {
+    __shared__ int s_m[3][4];
     int r = 0, lo = 0, up = 0;
-    if (i < n) {
-        r = rp[i + 1] - rp[i];
-        lo = dg[i] - rp[i];
-        up = rp[i + 1] - dg[i] - 1;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) {
+        r = max(r, rp[i + 1] - rp[i]);
+        lo = max(lo, dg[i] - rp[i]);
+        up = max(up, rp[i + 1] - dg[i] - 1);
     }
-    // wave maxima first: one atomic per wave and extent
     for (int o = 32; o > 0; o >>= 1) {
         r = max(r, __shfl_xor(r, o, 64));
         lo = max(lo, __shfl_xor(lo, o, 64));
         up = max(up, __shfl_xor(up, o, 64));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(ext, r);
-        atomicMax(ext + 1, lo);
-        atomicMax(ext + 2, up);
+        s_m[0][w] = r;
+        s_m[1][w] = lo;
+        s_m[2][w] = up;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int* m = s_m[threadIdx.x];
+        atomicMax(ext + threadIdx.x, max(max(m[0], m[1]), max(m[2], m[3])));
     }
 }
 
@@ -745,7 +759,8 @@ void analyse(Ext* e) {
 
     // longest row, longest L part and longest U part (device reduction)
     PILU_HIP(hipMemsetAsync(e->err.get() + kErrExtent, 0, 3 * sizeof(int), s));
-    k_row_extents<<<grid_for(n, 256), 256, 0, s>>>(n, e->rp.get(), e->dg.get(), e->err.get() + kErrExtent);
+    k_row_extents<<<std::min(grid_for(n, 256), 1024), 256, 0, s>>>(n, e->rp.get(), e->dg.get(),
+                                                                 e->err.get() + kErrExtent);
     PILU_HIP(hipGetLastError());
     int ext[3] = {0, 0, 0};
     PILU_HIP(hipMemcpyAsync(ext, e->err.get() + kErrExtent, sizeof(ext), hipMemcpyDeviceToHost, s));
