@@ -129,11 +129,13 @@ __device__ __forceinline__ int take_ticket(int* ctr) {
     return s_ticket;
 }
 
-__device__ __forceinline__ void retire_ticket(int* ctr) {
+// nblocks: workgroups that take tickets from ctr (the whole grid unless a
+// launch splits its workgroups over several ticket counters).
+__device__ __forceinline__ void retire_ticket(int* ctr, int nblocks = -1) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int done = __hip_atomic_fetch_add(&ctr[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (done == static_cast<int>(gridDim.x) - 1) {
+        if (done == (nblocks < 0 ? static_cast<int>(gridDim.x) : nblocks) - 1) {
             __hip_atomic_store(&ctr[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctr[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }

@@ -83,12 +83,13 @@ __global__ void k_iota(int n, int* __restrict__ p) {
 // restricts the graph to dependencies inside the row's block (local levels
 // of the block-local sweeps).
 // ---------------------------------------------------------------------
+// The body of one direction's analysis for a workgroup taking its tickets
+// from ctr; nblocks: the workgroups that share ctr.
 template <bool UPPER, int G>
-__global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict__ rp,
-                                                   const int* __restrict__ ci,
-                                                   const int* __restrict__ dg, int* level,
-                                                   int* maxlev, int* ctr, int* err, uint64_t wd_limit,
-                                                   const int* __restrict__ block_of) {
+__device__ __forceinline__ void levels_body(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                            const int* __restrict__ dg, int* level, int* maxlev, int* ctr,
+                                            int* err, uint64_t wd_limit, const int* __restrict__ block_of,
+                                            int nblocks) {
     constexpr int R = 64 / G;              // rows per wave
     constexpr int ROWS = (kBlock / 64) * R;  // rows per workgroup
     const int ticket = take_ticket(ctr);
@@ -237,7 +238,32 @@ __global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict_
         top = o > top ? o : top;
     }
     if (lane == 0 && top >= 0) atomicMax(maxlev, top);
-    retire_ticket(ctr);
+    retire_ticket(ctr, nblocks);
+}
+
+template <bool UPPER, int G>
+__global__ __launch_bounds__(kBlock) void k_levels(int n, const int* __restrict__ rp,
+                                                   const int* __restrict__ ci,
+                                                   const int* __restrict__ dg, int* level,
+                                                   int* maxlev, int* ctr, int* err, uint64_t wd_limit,
+                                                   const int* __restrict__ block_of) {
+    levels_body<UPPER, G>(n, rp, ci, dg, level, maxlev, ctr, err, wd_limit, block_of, static_cast<int>(gridDim.x));
+}
+
+// Both directions in one launch, so that the two dependency chains run at
+// the same time: even workgroups analyse L (tickets ctr[0..1], maxlev[0]),
+// odd ones U (ctr[2..3], maxlev[1]); nb workgroups per direction.  Each
+// direction's rows wait only on rows of lower tickets of the same
+// direction, taken by workgroups dispatched before them.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_levels_lu(int n, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci,
+                                                      const int* __restrict__ dg, int* level_l, int* level_u,
+                                                      int* maxlev, int* ctr, int* err, uint64_t wd_limit, int nb) {
+    if (blockIdx.x & 1)
+        levels_body<true, G>(n, rp, ci, dg, level_u, maxlev + 1, ctr + 2, err, wd_limit, nullptr, nb);
+    else
+        levels_body<false, G>(n, rp, ci, dg, level_l, maxlev, ctr, err, wd_limit, nullptr, nb);
 }
 
 // ---------------------------------------------------------------------

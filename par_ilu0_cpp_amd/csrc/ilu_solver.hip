@@ -109,7 +109,7 @@ uint64_t watchdog_ticks(double seconds) {
 
 // Error-word slots of IluSolver::Ext::err.
 enum ErrSlot { kErrFactor = 0, kErrSweep = 1, kErrSorted = 2, kErrDiag = 3, kErrMaxLev = 4, kErrRowPtr = 5,
-               kErrLevels = 6, kErrPivotRow = 7, kErrExtent = 8 /* 3 words */ };
+               kErrLevels = 6, kErrPivotRow = 7, kErrExtent = 8 /* 3 words */, kErrMaxLevLU = 12 /* 2 words */ };
 
 void check_sweep_errors(lljbash::IluSolver::Ext* e) {
     int v = 0;
@@ -225,24 +225,11 @@ void launch_levels(Ext* e, const int* block_of, int* level, int* maxlev, uint64_
     else levels_g<UPPER, 64>(e, block_of, level, maxlev, wd);
 }
 
-// Levels of one dependency graph and the rows sorted by (level, row).
-int build_schedule(Ext* e, bool upper, DevBuf<int>& perm, DevBuf<int>& lev_off) {
+// Rows sorted by (level, row) and each level's first position, from the
+// levels in `level` (nlev levels).
+void sort_by_level(Ext* e, const int* level, int nlev, DevBuf<int>& perm, DevBuf<int>& lev_off) {
     const int n = e->n;
     hipStream_t s = e->stream;
-    int* level = e->level.get();
-    int* maxlev = e->err.get() + kErrMaxLev;
-    int* ctr = e->ctr.get();
-    const uint64_t wd = watchdog_ticks(e->watchdog_s);
-    k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level, -1);
-    PILU_HIP(hipMemsetAsync(maxlev, 0, sizeof(int), s));
-    (void)ctr;
-    if (upper) launch_levels<true>(e, nullptr, level, maxlev, wd);
-    else launch_levels<false>(e, nullptr, level, maxlev, wd);
-    PILU_HIP(hipGetLastError());
-    const int nlev = read_int(maxlev, s) + 1;
-    if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
-        throw std::runtime_error("level analysis: dependency wait timed out (watchdog)");
-
     DevBuf<int> keys_out(n), rows(n);
     k_iota<<<grid_for(n, 256), 256, 0, s>>>(n, rows.get());
     perm.reserve(n);
@@ -252,8 +239,8 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm, DevBuf<int>& lev_off) 
     PILU_HIP(rocprim::radix_sort_pairs(nullptr, bytes, level, keys_out.get(), rows.get(), perm.get(),
                                        static_cast<unsigned>(n), 0u, end_bit, s));
     e->sort_tmp.reserve(bytes);
-    PILU_HIP(rocprim::radix_sort_pairs(e->sort_tmp.get(), bytes, level, keys_out.get(), rows.get(),
-                                       perm.get(), static_cast<unsigned>(n), 0u, end_bit, s));
+    PILU_HIP(rocprim::radix_sort_pairs(e->sort_tmp.get(), bytes, level, keys_out.get(), rows.get(), perm.get(),
+                                       static_cast<unsigned>(n), 0u, end_bit, s));
     lev_off.reserve(static_cast<size_t>(nlev) + 1);
     // every level 0..nlev-1 holds a row (a row's level is one more than its
     // deepest dependency's); the fill only keeps the array defined if not
@@ -261,7 +248,41 @@ int build_schedule(Ext* e, bool upper, DevBuf<int>& perm, DevBuf<int>& lev_off) 
     k_level_starts<<<grid_for(n, 256), 256, 0, s>>>(n, nlev, keys_out.get(), lev_off.get());
     PILU_HIP(hipGetLastError());
     PILU_HIP(hipStreamSynchronize(s));
-    return nlev;
+}
+
+// Levels of both dependency graphs in one launch (the two chains run at the
+// same time) and the rows of each sorted by (level, row).
+void build_schedules(Ext* e) {
+    const int n = e->n;
+    hipStream_t s = e->stream;
+    DevBuf<int> level_u(n);
+    int* level_l = e->level.get();
+    int* maxlev = e->err.get() + kErrMaxLevLU;  // [0] L, [1] U
+    const uint64_t wd = watchdog_ticks(e->watchdog_s);
+    k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level_l, -1);
+    k_fill_int<<<grid_for(n, 256 * 8), 256, 0, s>>>(n, level_u.get(), -1);
+    PILU_HIP(hipMemsetAsync(maxlev, 0, 2 * sizeof(int), s));
+    const double avg = n > 0 ? static_cast<double>(e->nnz - n) / (2.0 * n) : 0.0;
+    auto run = [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        constexpr int rows = (kBlock / 64) * (64 / G);
+        const int nb = grid_for(n, rows);
+        k_levels_lu<G><<<2 * nb, kBlock, 0, s>>>(n, e->rp.get(), e->ci.get(), e->dg.get(), level_l, level_u.get(),
+                                                 maxlev, e->ctr.get(), e->err.get() + kErrLevels, wd, nb);
+    };
+    if (avg <= 8) run(std::integral_constant<int, 1>{});
+    else if (avg <= 32) run(std::integral_constant<int, 8>{});
+    else run(std::integral_constant<int, 64>{});
+    PILU_HIP(hipGetLastError());
+    int ml[2] = {0, 0};
+    PILU_HIP(hipMemcpyAsync(ml, maxlev, sizeof(ml), hipMemcpyDeviceToHost, s));
+    PILU_HIP(hipStreamSynchronize(s));
+    if (read_int(e->err.get() + kErrLevels, s) & kErrTimeout)
+        throw std::runtime_error("level analysis: dependency wait timed out (watchdog)");
+    e->nlev_l = ml[0] + 1;
+    e->nlev_u = ml[1] + 1;
+    sort_by_level(e, level_l, e->nlev_l, e->perm_l, e->lvoff_l);
+    sort_by_level(e, level_u.get(), e->nlev_u, e->perm_u, e->lvoff_u);
 }
 
 __global__ void k_block_keys(int n, const int* __restrict__ block_of, const int* __restrict__ pos,
@@ -772,10 +793,8 @@ void analyse(Ext* e) {
     pt.mark("pattern checks, diagonal, row extents");
 
     (void)device_scratch();  // the no-allocation apply's scratch on this device
-    e->nlev_l = build_schedule(e, false, e->perm_l, e->lvoff_l);
-    pt.mark("L level schedule");
-    e->nlev_u = build_schedule(e, true, e->perm_u, e->lvoff_u);
-    pt.mark("U level schedule");
+    build_schedules(e);
+    pt.mark("L and U level schedules");
 
     e->vin.reserve(e->nnz);
     e->fac.reserve(e->nnz);
