@@ -264,6 +264,91 @@ __global__ __launch_bounds__(kSymWaves * 64) void k_symbolic(
     if (!a_pivots) retire_ticket(ctr);
 }
 
+// The rows-from-A form for short rows with G lanes per row.  With lof <= 1
+// and every row of A sorted and holding its diagonal, row i of F is A's row
+// i at level 0 plus, for lof = 1, the entries right of k of A's row k for
+// every k < i in A's row i, at level 1: such a candidate can only ever get
+// level 1, so the first-found rule has nothing to decide.  The group puts
+// the candidates in LDS, drops every candidate whose column has a level-0
+// or an earlier candidate, and ranks the rest by column into the row's
+// pool slot.  err bit 2: a row has more than CAP candidates (the caller
+// then takes the general form with long rows).
+template <int G, int CAP>
+__global__ __launch_bounds__(256) void k_symbolic_a(int n, const int* __restrict__ arp, const int* __restrict__ aci,
+                                                    int lof, uint32_t* __restrict__ pool, long* __restrict__ row_off,
+                                                    int* __restrict__ row_cnt, int* err) {
+    constexpr int GROUPS = 256 / G;
+    __shared__ uint32_t s_c[GROUPS][CAP];
+    __shared__ unsigned char s_keep[GROUPS][CAP];
+    const int grp = threadIdx.x / G, t = threadIdx.x % G;
+    uint32_t* cand = s_c[grp];
+    unsigned char* keep = s_keep[grp];
+    for (long i0 = static_cast<long>(blockIdx.x) * GROUPS; i0 < n; i0 += static_cast<long>(gridDim.x) * GROUPS) {
+        const long ii = i0 + grp;
+        if (ii >= n) continue;  // the whole group (a group lies inside one wave)
+        const int i = static_cast<int>(ii);
+        const int ab = arp[i];
+        const int r0 = arp[i + 1] - ab;
+        bool over = r0 > CAP;
+        int m = 0;
+        if (!over) {
+            for (int x = t; x < r0; x += G) cand[x] = static_cast<uint32_t>(aci[ab + x]) << kLevBits;
+            m = r0;
+            for (int p = 0; lof >= 1 && p < r0; ++p) {
+                const int k = aci[ab + p];
+                if (k >= i) break;
+                const int kb = arp[k], ke = arp[k + 1];
+                int le = 0;  // entries of row k up to its diagonal
+                for (int x = kb + t; x < ke; x += G) le += aci[x] <= k;
+#pragma unroll
+                for (int o = G / 2; o > 0; o >>= 1) le += __shfl_xor(le, o, 64);
+                const int ub = kb + le, uc = ke - ub;
+                if (m + uc > CAP) {
+                    over = true;
+                    break;
+                }
+                for (int x = t; x < uc; x += G) cand[m + x] = (static_cast<uint32_t>(aci[ub + x]) << kLevBits) | 1u;
+                m += uc;
+            }
+        }
+        if (over) {
+            if (t == 0) atomicOr(err, 2);
+            continue;
+        }
+        wave_sync_lds();
+        // a candidate stays unless its column has a level-0 candidate or an
+        // earlier one of the same level
+        for (int a = t; a < m; a += G) {
+            const uint32_t ca = cand[a];
+            bool drop = false;
+            for (int b = 0; b < m; ++b) {
+                const uint32_t cb = cand[b];
+                drop |= b != a && (cb >> kLevBits) == (ca >> kLevBits) &&
+                        ((cb & kLevMask) < (ca & kLevMask) || ((cb & kLevMask) == (ca & kLevMask) && b < a));
+            }
+            keep[a] = drop ? 0 : 1;
+        }
+        wave_sync_lds();
+        const long off = static_cast<long>(i) * CAP;
+        int kept = 0;
+        for (int a = t; a < m; a += G) {
+            if (!keep[a]) continue;
+            const uint32_t ca = cand[a];
+            int rank = 0;
+            for (int b = 0; b < m; ++b) rank += keep[b] && (cand[b] >> kLevBits) < (ca >> kLevBits);
+            pool[off + rank] = ca;
+            ++kept;
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) kept += __shfl_xor(kept, o, 64);
+        if (t == 0) {
+            row_off[i] = off;
+            row_cnt[i] = kept;
+        }
+        wave_sync_lds();  // the next row reuses the group's buffers
+    }
+}
+
 // Lay rows out as CSR: columns, levels, values (A's values at the level-0
 // slots in A's entry order, 0 at fill slots) and nzmap.
 __global__ __launch_bounds__(256) void k_sym_emit(int n, const int* __restrict__ arp,
@@ -366,7 +451,10 @@ int symbolic_device(int n, const int* d_arp, const int* d_aci, const double* d_a
         check(hipMemsetAsync(misc.get(), 0, 8 * sizeof(int), s), "misc");
         check(hipMemsetAsync(top.get(), 0, sizeof(unsigned long long), s), "top");
         const int grid = a_pivots ? std::min(ceil_div(n, kSymWaves), kSymGridA) : ceil_div(n, kSymWaves);
-        if (short_rows)
+        if (a_pivots && short_rows)
+            k_symbolic_a<16, kSymCapShort><<<std::min(ceil_div(n, 16), 8192), 256, 0, s>>>(
+                n, d_arp, d_aci, lof, pool.get(), row_off.get(), row_cnt.get(), misc.get() + 2);
+        else if (short_rows)
             k_symbolic<kSymCapShort><<<grid, kSymWaves * 64, 0, s>>>(
                 n, d_arp, d_aci, lof, pool.get(), cap, top.get(), header.get(), row_off.get(), row_cnt.get(),
                 misc.get() + 2, misc.get(), watchdog_ticks(watchdog_s), a_pivots);
